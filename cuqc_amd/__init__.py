@@ -1,0 +1,316 @@
+"""cuqc_amd -- maximal gamma-quasi-clique mining on AMD MI355X (gfx950).
+
+Python bindings (ctypes) of ``libqcmine.so``, whose C ABI is declared in
+``include/qcmine.h``.  The reference (Mike12041204/cuQC) is driven by its
+command line ``cuQC graph_file gamma min_size output_file scheduling_toggle``
+(main.cu:416-519); :func:`mine` takes the same arguments, and the ``qcmine``
+binary next to this file keeps that command line.
+
+The library needs a gfx950 device: every mining call raises :class:`QCError`
+when none is present.  There is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+__all__ = [
+    "QCError", "Graph", "Result", "mine", "mine_batch", "remove_nonmax", "device_available",
+    "library_path", "load_library",
+]
+
+_PKG = Path(__file__).resolve().parent
+_LIBPATH = _PKG / "libqcmine.so"
+
+QC_OK, QC_EINVAL, QC_EIO, QC_ENOMEM, QC_EOVERFLOW, QC_EDEVICE, QC_EEMPTY = range(7)
+_MODES = {"plain": 0, "persistent": 1}
+_ORDERS = {"fifo": 0, "cost": 1}
+
+
+class QCError(RuntimeError):
+    """A library call failed; ``code`` is the qc_status value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[qc_status {code}] {msg}")
+        self.code = code
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("min_size", C.c_int32), ("scheduling", C.c_int32), ("mode", C.c_int32),
+        ("order", C.c_int32), ("workgroups", C.c_int32), ("lds_tile", C.c_int32), ("split_depth", C.c_int32),
+        ("reserved0", C.c_int32), ("task_capacity", C.c_uint64), ("clique_capacity", C.c_uint64),
+        ("cost_model", C.c_void_p),
+    ]
+
+
+_STAT_U64 = ["tasks", "branches", "children", "raw_cliques", "max_cliques", "sum_t", "sum_t_written",
+             "sum_clique_len", "root_size", "levels", "large_tasks"]
+
+
+class _Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("reserved", C.c_uint64 * 5)] + [
+        ("t_init_ms", C.c_double), ("t_enum_ms", C.c_double), ("t_nonmax_ms", C.c_double), ("t_kernel_ms", C.c_double)]
+
+
+_lib = None
+
+
+def library_path() -> Path:
+    return _LIBPATH
+
+
+def load_library() -> C.CDLL:
+    """Load libqcmine.so (built in-tree by ``python -m cuqc_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIBPATH.exists():
+        raise QCError(QC_EDEVICE, f"{_LIBPATH} is not built; run `python -m cuqc_amd.build`")
+    L = C.CDLL(str(_LIBPATH))
+    P, PP = C.c_void_p, C.POINTER(C.c_void_p)
+    u64p = C.POINTER(C.c_uint64)
+    sig = {
+        "qc_graph_load_serialized": (C.c_int, [C.c_char_p, PP]),
+        "qc_graph_load_adjlist": (C.c_int, [C.c_char_p, PP]),
+        "qc_graph_write_serialized": (C.c_int, [P, C.c_char_p]),
+        "qc_graph_load_binary": (C.c_int, [C.c_char_p, PP]),
+        "qc_graph_write_binary": (C.c_int, [P, C.c_char_p]),
+        "qc_graph_from_csr": (C.c_int, [C.c_int32, P, P, PP]),
+        "qc_graph_generate": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, PP, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int32)]),
+        "qc_graph_csr": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
+        "qc_graph_checksum": (C.c_uint64, [P]),
+        "qc_graph_free": (None, [P]),
+        "qc_mine": (C.c_int, [P, C.POINTER(_Params), PP]),
+        "qc_mine_batch": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(_Params), C.c_int32, PP]),
+        "qc_result_view": (C.c_int, [P, u64p, C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
+        "qc_result_raw": (C.c_int, [P, u64p, C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
+        "qc_result_stats": (C.c_int, [P, C.POINTER(_Stats)]),
+        "qc_result_write": (C.c_int, [P, C.c_char_p]),
+        "qc_result_write_raw": (C.c_int, [P, C.c_char_p]),
+        "qc_result_free": (None, [P]),
+        "qc_remove_nonmax": (C.c_int, [C.c_uint64, P, P, PP]),
+        "qc_last_error": (C.c_char_p, []),
+        "qc_version": (C.c_char_p, []),
+        "qc_device_available": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, ok=(QC_OK,)) -> int:
+    if rc not in ok:
+        raise QCError(rc, load_library().qc_last_error().decode(errors="replace"))
+    return rc
+
+
+def device_available() -> bool:
+    return bool(load_library().qc_device_available())
+
+
+def _sets_from(count: int, off, verts) -> list[np.ndarray]:
+    if count == 0:
+        return []
+    o = np.ctypeslib.as_array(off, shape=(count + 1,)).copy()
+    v = np.ctypeslib.as_array(verts, shape=(int(o[-1]),)).copy() if o[-1] else np.zeros(0, np.uint32)
+    return [v[o[i]:o[i + 1]] for i in range(count)]
+
+
+class Graph:
+    """An undirected graph held by the library (symmetric CSR, sorted lists)."""
+
+    def __init__(self, handle: int, gamma: float | None = None, min_size: int | None = None):
+        self._h = C.c_void_p(handle)
+        self.gamma = gamma
+        self.min_size = min_size
+
+    # -- constructors ---------------------------------------------------
+    @classmethod
+    def from_csr(cls, offsets, adj) -> "Graph":
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        a = np.ascontiguousarray(adj, dtype=np.uint32)
+        h = C.c_void_p()
+        _check(load_library().qc_graph_from_csr(len(off) - 1, off.ctypes.data, a.ctypes.data if a.size else None,
+                                                C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, n: int, edges) -> "Graph":
+        e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+        e = e[e[:, 0] != e[:, 1]]
+        both = np.concatenate([e, e[:, ::-1]])
+        both = np.unique(both, axis=0) if len(both) else both.reshape(0, 2)
+        off = np.zeros(n + 1, np.uint64)
+        if len(both):
+            np.add.at(off, both[:, 0] + 1, 1)
+        off = np.cumsum(off).astype(np.uint64)
+        return cls.from_csr(off, both[:, 1].astype(np.uint32) if len(both) else np.zeros(0, np.uint32))
+
+    @classmethod
+    def generate(cls, config: int, seed: int, instance: int = 0) -> "Graph":
+        h, g, m = C.c_void_p(), C.c_double(), C.c_int32()
+        _check(load_library().qc_graph_generate(config, seed, instance, C.byref(h), C.byref(g), C.byref(m)))
+        return cls(h.value, g.value, m.value)
+
+    @classmethod
+    def load_serialized(cls, path: str | os.PathLike) -> "Graph":
+        h = C.c_void_p()
+        _check(load_library().qc_graph_load_serialized(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_adjlist(cls, path: str | os.PathLike) -> "Graph":
+        h = C.c_void_p()
+        _check(load_library().qc_graph_load_adjlist(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_binary(cls, path: str | os.PathLike) -> "Graph":
+        h = C.c_void_p()
+        _check(load_library().qc_graph_load_binary(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    # -- views ----------------------------------------------------------
+    def csr(self) -> tuple[np.ndarray, np.ndarray]:
+        n = C.c_int32()
+        o = C.POINTER(C.c_uint64)()
+        a = C.POINTER(C.c_uint32)()
+        _check(load_library().qc_graph_csr(self._h, C.byref(n), C.byref(o), C.byref(a)))
+        off = np.ctypeslib.as_array(o, shape=(n.value + 1,)).copy()
+        adj = np.ctypeslib.as_array(a, shape=(int(off[-1]),)).copy() if off[-1] else np.zeros(0, np.uint32)
+        return off, adj
+
+    @property
+    def n(self) -> int:
+        n = C.c_int32()
+        _check(load_library().qc_graph_csr(self._h, C.byref(n), None, None))
+        return n.value
+
+    def checksum(self) -> int:
+        return int(load_library().qc_graph_checksum(self._h))
+
+    def write_serialized(self, path) -> None:
+        _check(load_library().qc_graph_write_serialized(self._h, os.fsencode(path)))
+
+    def write_binary(self, path) -> None:
+        _check(load_library().qc_graph_write_binary(self._h, os.fsencode(path)))
+
+    def close(self) -> None:
+        if self._h:
+            load_library().qc_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Result:
+    """Maximal sets (ascending, lexicographic order), raw sets and statistics."""
+
+    maximal: list[np.ndarray]
+    raw: list[np.ndarray]
+    stats: dict
+    status: int
+
+    def write(self, path) -> None:
+        write_sets(self.maximal, path)
+
+    def maximal_sorted_tuples(self) -> list[tuple[int, ...]]:
+        return [tuple(int(x) for x in s) for s in self.maximal]
+
+
+def write_sets(sets: Sequence[Sequence[int]], path) -> None:
+    """OutputOneSet format (main.cu:4641-4651)."""
+    with open(path, "w") as f:
+        for s in sets:
+            f.write(f"{len(s)} " + "".join(f"{int(x)} " for x in s) + "\n")
+
+
+def _result_from(h: C.c_void_p, rc: int) -> Result:
+    L = load_library()
+    try:
+        cnt, off, v = C.c_uint64(), C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint32)()
+        _check(L.qc_result_view(h, C.byref(cnt), C.byref(off), C.byref(v)))
+        maximal = _sets_from(cnt.value, off, v)
+        _check(L.qc_result_raw(h, C.byref(cnt), C.byref(off), C.byref(v)))
+        raw = _sets_from(cnt.value, off, v)
+        st = _Stats()
+        _check(L.qc_result_stats(h, C.byref(st)))
+        stats = {n: int(getattr(st, n)) for n in _STAT_U64}
+        stats.update(t_init_ms=st.t_init_ms, t_enum_ms=st.t_enum_ms, t_nonmax_ms=st.t_nonmax_ms,
+                     t_kernel_ms=st.t_kernel_ms)
+        return Result(maximal, raw, stats, rc)
+    finally:
+        L.qc_result_free(h)
+
+
+def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
+            cost_model=None, split_depth=0) -> _Params:
+    p = _Params()
+    p.gamma = float(gamma)
+    p.min_size = int(min_size)
+    p.scheduling = int(scheduling)
+    p.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
+    p.order = _ORDERS[order] if isinstance(order, str) else int(order)
+    p.lds_tile = int(lds_tile)
+    p.workgroups = int(workgroups)
+    p.split_depth = int(split_depth)
+    p.task_capacity = int(task_capacity)
+    p.clique_capacity = int(clique_capacity)
+    p.cost_model = cost_model
+    return p
+
+
+def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode: str = "plain",
+         order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
+         clique_capacity: int = 0, cost_model=None, split_depth: int = 0) -> Result:
+    """Mine maximal gamma-quasi-cliques of size >= min_size (search() + RemoveNonMax()).
+
+    An empty result is returned (status QC_EEMPTY) rather than raised.
+    """
+    p = _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
+                cost_model, split_depth)
+    h = C.c_void_p()
+    rc = _check(load_library().qc_mine(graph._h, C.byref(p), C.byref(h)), ok=(QC_OK, QC_EEMPTY))
+    return _result_from(h, rc)
+
+
+def mine_batch(graphs: Sequence[Graph], gammas: Sequence[float], min_sizes: Sequence[int], *, scheduling: int = 0,
+               order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
+               clique_capacity: int = 0, cost_model=None, split_depth: int = 0) -> list[Result]:
+    """Many instances through one persistent launch."""
+    n = len(graphs)
+    ps = (_Params * n)()
+    for i in range(n):
+        ps[i] = _params(gammas[i], min_sizes[i], scheduling, "persistent", order, lds_tile, workgroups,
+                        task_capacity, clique_capacity, cost_model, split_depth)
+    hs = (C.c_void_p * n)(*[g._h for g in graphs])
+    out = (C.c_void_p * n)()
+    _check(load_library().qc_mine_batch(hs, ps, n, out))
+    return [_result_from(C.c_void_p(out[i]), QC_OK) for i in range(n)]
+
+
+def remove_nonmax(sets: Sequence[Sequence[int]]) -> list[np.ndarray]:
+    """RemoveNonMax on the device for caller-provided raw sets."""
+    lens = np.array([len(s) for s in sets], np.uint64)
+    off = np.zeros(len(sets) + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    v = np.concatenate([np.asarray(s, np.uint32) for s in sets]) if len(sets) else np.zeros(0, np.uint32)
+    h = C.c_void_p()
+    rc = _check(load_library().qc_remove_nonmax(len(sets), off.ctypes.data, v.ctypes.data if v.size else None,
+                                                C.byref(h)), ok=(QC_OK, QC_EEMPTY))
+    return _result_from(h, rc).maximal

@@ -1,0 +1,952 @@
+// expand.hip -- gfx950 expansion of quasi-clique task records.
+//
+// One 64-lane wavefront owns one task record at a time and runs the whole
+// per-task step of the reference host path (h_expand_level body,
+// main.cu:1060-1181): lookahead, the sequential branch loop with
+// remove-one-vertex, add-one-vertex with diameter and degree pruning,
+// critical-vertex pruning, the clique test and the Q-ordered child write.
+//
+// Adjacency.  Every task is a subset of the root task, so vertex ids are
+// root-local (0..R-1) and the 1-hop / 2-hop relations are kept as R x R bit
+// matrices in HBM (A1, A2).  A task of t <= TL vertices is staged in LDS
+// together with its own t x t sub-matrices (row = position in the task
+// record as loaded), so each pruning step is a row AND mask popcount; larger
+// tasks run from a per-wave HBM slab and address A1/A2 directly.  Where the
+// reference scans adjacency lists through its id->position map
+// (vertex_order_map), this code tests bits: the two are the same relation.
+//
+// Semantics follow the host column of the reference (main.cu:1349-2198),
+// including the two degree-update branches of a degree-pruning round
+// (recompute from the remaining set when it is smaller than the removed set,
+// else decrement via the removed set, main.cu:1775-1807) and the round
+// structure that follows critical-vertex pruning when no candidate is
+// critical-adjacent: the reference then leaves its id->position map empty
+// (main.cu:1575-1602), so a recompute round sets every exdeg to 0 and a
+// decrement round changes nothing.  Both effects are reproduced.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qc_device.h"
+
+namespace qcdev {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+// inclusive prefix sum across the wave
+__device__ __forceinline__ long long wave_incl_scan64(long long v) {
+    int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        long long u = __shfl_up(v, o, WAVE);
+        if (l >= o) v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }
+
+__device__ __forceinline__ bool gbit(const uint64_t* M, int W, int row, int col) {
+    return (M[(size_t)row * W + (col >> 6)] >> (col & 63)) & 1ull;
+}
+
+// ---------------------------------------------------------------------------
+// workspace
+// ---------------------------------------------------------------------------
+struct WS {
+    int *pid, *plab, *pin, *pex, *pl2;           // parent record
+    int *cid, *cpp, *clab, *cin, *cex, *cl2;     // child under construction
+    int *rem, *rmv, *cind, *adjc;                // lists (4 x T2 ints)
+    uint64_t *khi, *klo;                         // sort keys, alias the lists
+    uint64_t *M1, *M2;                           // local matrices (LDS mode)
+    uint64_t *mk0, *mk1;                         // masks (W words each)
+    int T2;                                      // capacity rounded to 2^k
+};
+
+__host__ __device__ constexpr uint32_t pow2_at_least(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// bytes of a workspace for `tcap` vertices; `mat` = build local matrices,
+// `mwords` = words per mask (ceil(tcap/64) in LDS mode, rw in root space)
+__host__ __device__ constexpr size_t ws_bytes(uint32_t tcap, bool mat, uint32_t mwords) {
+    uint32_t T2 = pow2_at_least(tcap < 64 ? 64 : tcap);
+    uint32_t W = (tcap + 63) / 64;
+    size_t b = 0;
+    b += (size_t)11 * T2 * 4;        // parent + child arrays
+    b += (size_t)4 * T2 * 4;         // lists / keys
+    b = (b + 15) & ~(size_t)15;
+    if (mat) b += (size_t)2 * tcap * W * 8;
+    b += (size_t)2 * mwords * 8;
+    return b;
+}
+
+__device__ inline WS make_ws(char* base, uint32_t tcap, bool mat, uint32_t mwords) {
+    WS w;
+    uint32_t T2 = pow2_at_least(tcap < 64 ? 64 : tcap);
+    uint32_t W = (tcap + 63) / 64;
+    int* ip = (int*)base;
+    w.pid = ip; ip += T2;
+    w.plab = ip; ip += T2;
+    w.pin = ip; ip += T2;
+    w.pex = ip; ip += T2;
+    w.pl2 = ip; ip += T2;
+    w.cid = ip; ip += T2;
+    w.cpp = ip; ip += T2;
+    w.clab = ip; ip += T2;
+    w.cin = ip; ip += T2;
+    w.cex = ip; ip += T2;
+    w.cl2 = ip; ip += T2;
+    w.rem = ip; ip += T2;
+    w.rmv = ip; ip += T2;
+    w.cind = ip; ip += T2;
+    w.adjc = ip; ip += T2;
+    w.khi = (uint64_t*)w.rem;
+    w.klo = (uint64_t*)w.cind;
+    size_t b = (size_t)15 * T2 * 4;
+    b = (b + 15) & ~(size_t)15;
+    uint64_t* up = (uint64_t*)(base + b);
+    if (mat) {
+        w.M1 = up; up += (size_t)tcap * W;
+        w.M2 = up; up += (size_t)tcap * W;
+    } else {
+        w.M1 = w.M2 = nullptr;
+    }
+    w.mk0 = up; up += mwords;
+    w.mk1 = up; up += mwords;
+    w.T2 = (int)T2;
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// per-task state
+// ---------------------------------------------------------------------------
+struct LStats {
+    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large;
+};
+
+struct Ctx {
+    const ExpandArgs* a;
+    WS w;
+    const uint64_t* M1;   // matrix used for tests (local or root space)
+    const uint64_t* M2;
+    int W;                // words per row of M1/M2 = words per mask
+    int lane;
+    LStats st;
+    uint32_t depth;
+    uint32_t cap;         // largest task the workspace holds
+};
+
+__device__ __forceinline__ int md(const Ctx& c, int i) {
+    // index stays below mindeg_n by construction of the table size
+    return c.a->g.mindeg[i];
+}
+__device__ __forceinline__ int get_mindeg(const Ctx& c, int s) {
+    int ms = c.a->g.min_size;
+    return s < ms ? md(c, ms) : md(c, s);
+}
+
+// main.cu:2153-2173
+__device__ __forceinline__ bool cand_valid(const Ctx& c, int in, int ex, int m, int ub, int lb, int med) {
+    int tot = in + ex;
+    if (tot < md(c, c.a->g.min_size)) return false;
+    if (tot < get_mindeg(c, m + ex + 1)) return false;
+    if (tot < med) return false;
+    if (in + ub - 1 < md(c, m + lb)) return false;
+    if (tot < get_mindeg(c, m + lb)) return false;
+    return true;
+}
+// main.cu:2175-2198
+__device__ __forceinline__ bool vert_ext(const Ctx& c, int in, int ex, int m, int ub, int lb, int med) {
+    int tot = in + ex;
+    if (tot < md(c, c.a->g.min_size)) return false;
+    if (tot < get_mindeg(c, m + ex)) return false;
+    if (tot < med) return false;
+    if (ex == 0 && in < get_mindeg(c, m + ex)) return false;
+    if (in + ub < md(c, m + ub)) return false;
+    if (tot < get_mindeg(c, m + lb)) return false;
+    return true;
+}
+
+__device__ __forceinline__ int rowpop(const Ctx& c, const uint64_t* M, int row, const uint64_t* mask) {
+    const uint64_t* r = M + (size_t)row * c.W;
+    int s = 0;
+    for (int w = 0; w < c.W; w++) s += __popcll(r[w] & mask[w]);
+    return s;
+}
+
+__device__ inline void clear_mask(Ctx& c, uint64_t* mk) {
+    for (int w = c.lane; w < c.W; w += WAVE) mk[w] = 0;
+    wsync();
+}
+
+__device__ inline void set_mask_bit(uint64_t* mk, int b) {
+    atomicOr((unsigned long long*)&mk[b >> 6], 1ull << (b & 63));
+}
+
+// descending sort of ints a[0..n) (bitonic, padded with INT_MIN)
+__device__ void sort_desc_int(Ctx& c, int* a, int n) {
+    if (n <= 1) return;
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (int i = n + c.lane; i < n2; i += WAVE) a[i] = INT32_MIN;
+    wsync();
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = c.lane; i < n2; i += WAVE) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    int x = a[i], y = a[ixj];
+                    bool desc = (i & k) == 0;
+                    if (desc ? (x < y) : (x > y)) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            wsync();
+        }
+    }
+}
+
+// ascending sort of 128-bit keys (hi, lo) over [0, n)
+__device__ void sort_keys(Ctx& c, uint64_t* hi, uint64_t* lo, int n) {
+    if (n <= 1) return;
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (int i = n + c.lane; i < n2; i += WAVE) {
+        hi[i] = ~0ull;
+        lo[i] = ~0ull;
+    }
+    wsync();
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = c.lane; i < n2; i += WAVE) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t xh = hi[i], xl = lo[i], yh = hi[ixj], yl = lo[ixj];
+                    bool gt = xh > yh || (xh == yh && xl > yl);
+                    bool asc = (i & k) == 0;
+                    if (asc == gt) {
+                        hi[i] = yh;
+                        lo[i] = yl;
+                        hi[ixj] = xh;
+                        lo[ixj] = xl;
+                    }
+                }
+            }
+            wsync();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LU bounds, main.cu:1857-1972.  Members are child positions [0, m);
+// cind[0..nc) holds candidate indegrees sorted descending.
+// ---------------------------------------------------------------------------
+__device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med) {
+    WS& w = c.w;
+    const ExpandArgs& a = *c.a;
+    int min_in = INT32_MAX, min_tot = INT32_MAX;
+    long long sum_in = 0;
+    for (int i = c.lane; i < m; i += WAVE) {
+        int in = w.cin[i], ex = w.cex[i];
+        min_in = min(min_in, in);
+        min_tot = min(min_tot, in + ex);
+        sum_in += in;
+    }
+    min_in = uni(wave_min(min_in));
+    min_tot = uni(wave_min(min_tot));
+    sum_in = wave_sum64(sum_in);
+    int min_in_ex = INT32_MAX;
+    for (int i = c.lane; i < m; i += WAVE)
+        if (w.cin[i] == min_in) min_in_ex = min(min_in_ex, w.cex[i]);
+    min_in_ex = uni(wave_min(min_in_ex));
+
+    bool invalid = false;
+    med = get_mindeg(c, m + 1);
+    if (min_in < md(c, m)) {
+        int lb0 = get_mindeg(c, m) - min_in;
+        // first x >= lb0 with x > min_in_ex || min_in + x >= mindeg[m + x]
+        int x = lb0;
+        for (;;) {
+            int y = x + c.lane;
+            bool stop = (y > min_in_ex) || (min_in + y >= md(c, m + y));
+            uint64_t bm = ballot(stop);
+            if (bm) {
+                x += __builtin_ctzll(bm);
+                break;
+            }
+            x += WAVE;
+        }
+        lb = x;
+        if (min_in + lb < md(c, m + lb)) {
+            lb = nc + 1;
+            invalid = true;
+        }
+        int tq = min_tot < a.g.ubq_n ? a.g.ubq[min_tot] : a.g.ubq[a.g.ubq_n - 1];
+        ub = tq + 1 - m;
+        if (ub > nc) ub = nc;
+        if (lb < ub) {
+            // P(idx) = sum of the idx largest candidate indegrees;
+            // cond(idx) = sum_in + P(idx) >= m * mindeg[m + idx]
+            // i0 = first idx in [lb, ub) with cond, else ub
+            long long carry = 0;
+            int first = -1, last = -1;
+            for (int base = 0; base <= ub; base += WAVE) {
+                int idx = base + c.lane;  // P(idx) = prefix over [0, idx)
+                long long v = (idx - 1 >= 0 && idx - 1 < nc && idx <= ub) ? (long long)w.cind[idx - 1] : 0;
+                long long inc = wave_incl_scan64(v) + carry;  // = P(idx)
+                carry = __shfl(inc, WAVE - 1, WAVE);
+                bool inr = idx <= ub;
+                bool cnd = inr && (sum_in + inc >= (long long)m * md(c, m + idx));
+                uint64_t bfirst = ballot(cnd && idx >= lb);
+                if (first < 0 && bfirst) first = base + __builtin_ctzll(bfirst);
+                uint64_t blast = ballot(cnd && idx >= lb);
+                if (blast) last = base + 63 - __builtin_clzll(blast);
+            }
+            first = uni(first);
+            last = uni(last);
+            if (first < 0) {
+                lb = ub + 1;
+                invalid = true;
+            } else {
+                // the reference's first loop stops at idx == ub even when
+                // cond(ub) is false; first > ub is impossible here
+                lb = first;
+                int tub = last > first ? last : first;
+                if (ub > tub) ub = tub;
+                if (lb > 1) med = get_mindeg(c, m + lb);
+            }
+        }
+    } else {
+        ub = nc;
+        int ms = a.g.min_size;
+        lb = m < ms ? ms - m : 0;
+    }
+    if (m + ub < a.g.min_size) invalid = true;
+    if (ub < 0 || ub < lb) invalid = true;
+    return invalid;
+}
+
+__device__ inline bool members_fail(Ctx& c, int m, int ub, int lb, int med) {
+    bool f = false;
+    for (int k = c.lane; k < m; k += WAVE)
+        if (!vert_ext(c, c.w.cin[k], c.w.cex[k], m, ub, lb, med)) f = true;
+    return ballot(f) != 0;
+}
+
+// ---------------------------------------------------------------------------
+// degree pruning, main.cu:1735-1855.  cind[0..nind) = candidate indegrees.
+// ---------------------------------------------------------------------------
+// map_empty: the reference's id->position map holds no entry during this call
+// (critical-vertex pruning without critical-adjacent candidates).
+__device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb, int& med, int nind,
+                               bool map_empty) {
+    WS& w = c.w;
+    sort_desc_int(c, w.cind, nind);
+    if (lu_bounds(c, m, nind, ub, lb, med)) return true;
+    if (members_fail(c, m, ub, lb, med)) return true;
+    // classify candidates, stable
+    int nrem = 0, nrmv = 0;
+    for (int base = m; base < tc; base += WAVE) {
+        int i = base + c.lane;
+        bool in = i < tc;
+        bool ok = in && w.clab[i] == 0 && cand_valid(c, w.cin[i], w.cex[i], m, ub, lb, med);
+        uint64_t bo = ballot(ok), bx = ballot(in && !ok);
+        if (ok) w.rem[nrem + lanes_below(bo)] = i;
+        if (in && !ok) w.rmv[nrmv + lanes_below(bx)] = i;
+        nrem += __popcll(bo);
+        nrmv += __popcll(bx);
+    }
+    wsync();
+    while (nrem > 0 && nrmv > 0) {
+        // degree update, main.cu:1775-1807
+        if (nrem < nrmv) {
+            if (map_empty) {
+                for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = 0;
+            } else {
+                clear_mask(c, w.mk0);
+                for (int k = c.lane; k < nrem; k += WAVE) set_mask_bit(w.mk0, w.cpp[w.rem[k]]);
+                wsync();
+                for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = rowpop(c, c.M1, w.cpp[i], w.mk0);
+            }
+        } else if (!map_empty) {
+            clear_mask(c, w.mk0);
+            for (int k = c.lane; k < nrmv; k += WAVE) set_mask_bit(w.mk0, w.cpp[w.rmv[k]]);
+            wsync();
+            for (int i = c.lane; i < tc; i += WAVE) w.cex[i] -= rowpop(c, c.M1, w.cpp[i], w.mk0);
+        }
+        wsync();
+        // indegrees of remaining candidates valid under the previous bounds
+        int nv = 0;
+        for (int base = 0; base < nrem; base += WAVE) {
+            int k = base + c.lane;
+            bool ok = false;
+            int in = 0;
+            if (k < nrem) {
+                int i = w.rem[k];
+                in = w.cin[i];
+                ok = cand_valid(c, in, w.cex[i], m, ub, lb, med);
+            }
+            uint64_t bo = ballot(ok);
+            if (ok) w.cind[nv + lanes_below(bo)] = in;
+            nv += __popcll(bo);
+        }
+        wsync();
+        sort_desc_int(c, w.cind, nv);
+        if (lu_bounds(c, m, nv, ub, lb, med)) return true;
+        if (members_fail(c, m, ub, lb, med)) return true;
+        // re-classify the remaining set under the new bounds (in place)
+        int kept = 0;
+        nrmv = 0;
+        for (int base = 0; base < nrem; base += WAVE) {
+            int k = base + c.lane;
+            bool in = k < nrem;
+            int i = in ? w.rem[k] : 0;
+            bool ok = in && cand_valid(c, w.cin[i], w.cex[i], m, ub, lb, med);
+            uint64_t bo = ballot(ok), bx = ballot(in && !ok);
+            wsync();
+            if (ok) w.rem[kept + lanes_below(bo)] = i;
+            if (in && !ok) w.rmv[nrmv + lanes_below(bx)] = i;
+            kept += __popcll(bo);
+            nrmv += __popcll(bx);
+            wsync();
+        }
+        nrem = kept;
+    }
+    // compaction: child[m + k] = child[rem[k]] (rem ascending, m + k <= rem[k])
+    for (int base = 0; base < nrem; base += WAVE) {
+        int k = base + c.lane;
+        int i = 0, id = 0, pp = 0, lab = 0, in = 0, ex = 0, l2 = 0;
+        if (k < nrem) {
+            i = w.rem[k];
+            id = w.cid[i];
+            pp = w.cpp[i];
+            lab = w.clab[i];
+            in = w.cin[i];
+            ex = w.cex[i];
+            l2 = w.cl2[i];
+        }
+        wsync();
+        if (k < nrem) {
+            int d = m + k;
+            w.cid[d] = id;
+            w.cpp[d] = pp;
+            w.clab[d] = lab;
+            w.cin[d] = in;
+            w.cex[d] = ex;
+            w.cl2[d] = l2;
+        }
+        wsync();
+    }
+    tc = tc - nc + nrem;
+    nc = nrem;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// critical vertex pruning, main.cu:1526-1707; returns 0, 1 or 2 like the
+// reference
+// ---------------------------------------------------------------------------
+__device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& lb, int& med) {
+    WS& w = c.w;
+    int thr = md(c, m + lb);
+    // mask of critical members
+    clear_mask(c, w.mk0);
+    bool any = false;
+    for (int k = c.lane; k < m; k += WAVE) {
+        if (w.cin[k] + w.cex[k] == thr && w.cex[k] > 0) {
+            set_mask_bit(w.mk0, w.cpp[k]);
+            any = true;
+        }
+    }
+    wsync();
+    // critical-adjacent candidates
+    int ncrit = 0;
+    if (ballot(any)) {
+        for (int base = m; base < tc; base += WAVE) {
+            int i = base + c.lane;
+            bool f = i < tc && rowpop(c, c.M1, w.cpp[i], w.mk0) > 0;
+            ncrit += __popcll(ballot(f));
+            if (f) w.clab[i] = 4;
+        }
+        wsync();
+    }
+    if (ncrit == 0) {
+        // every candidate passes the 2-hop filter; the map stays empty
+        int nind = 0;
+        for (int base = m; base < tc; base += WAVE) {
+            int i = base + c.lane;
+            bool in = i < tc;
+            uint64_t bo = ballot(in);
+            if (in) w.cind[nind + lanes_below(bo)] = w.cin[i];
+            nind += __popcll(bo);
+        }
+        wsync();
+        return degree_pruning(c, tc, nc, m, ub, lb, med, nind, true) ? 1 : 0;
+    }
+    // stable partition of [m, tc): label 4 first (perm in rmv, temp in cind)
+    {
+        int nf = 0, nn = 0;
+        for (int base = m; base < tc; base += WAVE) {
+            int i = base + c.lane;
+            bool in = i < tc;
+            bool f = in && w.clab[i] == 4;
+            uint64_t bf = ballot(f), bn = ballot(in && !f);
+            if (f) w.rmv[nf + lanes_below(bf)] = i;
+            if (in && !f) w.rmv[ncrit + nn + lanes_below(bn)] = i;
+            nf += __popcll(bf);
+            nn += __popcll(bn);
+        }
+        wsync();
+        int cnt = tc - m;
+        auto permute = [&](int* F) {
+            for (int k = c.lane; k < cnt; k += WAVE) w.cind[k] = F[w.rmv[k]];
+            wsync();
+            for (int k = c.lane; k < cnt; k += WAVE) F[m + k] = w.cind[k];
+            wsync();
+        };
+        permute(w.cid);
+        permute(w.cpp);
+        permute(w.clab);
+        permute(w.cin);
+        permute(w.cex);
+        permute(w.cl2);
+    }
+    // add the critical-adjacent vertices: degree updates and 2-hop counts
+    clear_mask(c, w.mk1);
+    for (int k = c.lane; k < ncrit; k += WAVE) set_mask_bit(w.mk1, w.cpp[m + k]);
+    wsync();
+    for (int i = c.lane; i < tc; i += WAVE) {
+        int pp = w.cpp[i];
+        int d1 = rowpop(c, c.M1, pp, w.mk1);
+        w.cin[i] += d1;
+        w.cex[i] -= d1;
+        w.adjc[i] = rowpop(c, c.M2, pp, w.mk1);
+    }
+    wsync();
+    bool fail = false;
+    for (int k = c.lane; k < m; k += WAVE)
+        if (w.adjc[k] != ncrit) fail = true;
+    for (int k = m + c.lane; k < m + ncrit; k += WAVE)
+        if (w.adjc[k] < ncrit - 1) fail = true;
+    if (ballot(fail)) return 2;
+    for (int k = m + c.lane; k < m + ncrit; k += WAVE) w.clab[k] = 1;
+    m += ncrit;
+    nc -= ncrit;
+    // diameter step around the added vertices
+    int nind = 0;
+    for (int base = m; base < tc; base += WAVE) {
+        int i = base + c.lane;
+        bool in = i < tc;
+        bool ok = in && w.adjc[i] == ncrit;
+        uint64_t bo = ballot(ok);
+        if (ok) w.cind[nind + lanes_below(bo)] = w.cin[i];
+        if (in && !ok) w.clab[i] = -1;
+        nind += __popcll(bo);
+    }
+    wsync();
+    return degree_pruning(c, tc, nc, m, ub, lb, med, nind, false) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// emission
+// ---------------------------------------------------------------------------
+__device__ void emit_clique(Ctx& c, const int* ids, int len) {
+    const ExpandArgs& a = *c.a;
+    unsigned long long ri = 0, si = 0;
+    if (c.lane == 0) {
+        ri = atomicAdd(&a.clq_ctr->items, 1ull);
+        si = atomicAdd(&a.clq_ctr->slots, (unsigned long long)len);
+    }
+    ri = __shfl(ri, 0, WAVE);
+    si = __shfl(si, 0, WAVE);
+    if (ri >= a.clq_ref_cap || si + len > a.clq_vert_cap) {
+        if (c.lane == 0) atomicOr(a.error, 1);
+        return;
+    }
+    for (int k = c.lane; k < len; k += WAVE) a.clq_verts[si + k] = a.g.orig[ids[k]];
+    if (c.lane == 0) {
+        CliqueRef r;
+        r.off = si;
+        r.len = (uint32_t)len;
+        r.pad = 0;
+        a.clq_refs[ri] = r;
+    }
+    c.st.raw++;
+    c.st.sum_clq += len;
+}
+
+// clique test, main.cu:1974-1995
+__device__ void check_for_clique(Ctx& c, int m) {
+    int req = md(c, m);
+    bool bad = false;
+    for (int k = c.lane; k < m; k += WAVE)
+        if (c.w.cin[k] < req) bad = true;
+    if (ballot(bad)) return;
+    emit_clique(c, c.w.cid, m);
+}
+
+static __device__ __forceinline__ uint32_t label_rank(int l) {
+    return l == 1 ? 0u : l == 2 ? 1u : l == 0 ? 2u : l == 3 ? 3u : 4u;
+}
+
+// Q order (main.cu:2059-2104) as an ascending 128-bit key
+__device__ __forceinline__ void q_key(int lab, int in, int ex, int l2, int id, uint64_t& hi, uint64_t& lo) {
+    hi = ((uint64_t)label_rank(lab) << 61) | ((uint64_t)(0x3FFFFFFF - in) << 30) | (uint64_t)(0x3FFFFFFF - ex);
+    lo = ((uint64_t)(0x7FFFFFFFu - (uint32_t)l2) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)id);
+}
+
+__device__ void write_child(Ctx& c, int tc, int mem) {
+    const ExpandArgs& a = *c.a;
+    WS& w = c.w;
+    for (int i = c.lane; i < tc; i += WAVE) q_key(w.clab[i], w.cin[i], w.cex[i], w.cl2[i], w.cid[i], w.khi[i], w.klo[i]);
+    wsync();
+    sort_keys(c, w.khi, w.klo, tc);
+    int s = (uint32_t)tc <= a.small_max ? 0 : 1;
+    unsigned long long ri = 0, si = 0;
+    if (c.lane == 0) {
+        ri = atomicAdd(&a.out_ctr[s].items, 1ull);
+        si = atomicAdd(&a.out_ctr[s].slots, (unsigned long long)tc);
+    }
+    ri = __shfl(ri, 0, WAVE);
+    si = __shfl(si, 0, WAVE);
+    if (ri >= a.out_ref_cap[s] || si + tc > a.out_vert_cap[s]) {
+        if (c.lane == 0) atomicOr(a.error, 1);
+        return;
+    }
+    Vrec* dst = a.out_verts[s] + si;
+    for (int i = c.lane; i < tc; i += WAVE) {
+        uint64_t hi = w.khi[i], lo = w.klo[i];
+        uint32_t rank = (uint32_t)(hi >> 61);
+        Vrec v;
+        v.id = (int32_t)(0xFFFFFFFFu - (uint32_t)lo);
+        v.label = rank == 0 ? 1 : rank == 1 ? 2 : rank == 2 ? 0 : rank == 3 ? 3 : -1;
+        v.indeg = 0x3FFFFFFF - (int32_t)((hi >> 30) & 0x3FFFFFFFull);
+        v.exdeg = 0x3FFFFFFF - (int32_t)(hi & 0x3FFFFFFFull);
+        dst[i] = v;
+    }
+    // slot bound of this child's own children (see TaskRef)
+    unsigned long long bnd = 0;
+    for (int i = mem + c.lane; i < tc; i += WAVE) {
+        unsigned long long x = (unsigned long long)a.g.d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
+        bnd += x < (unsigned long long)tc ? x : (unsigned long long)tc;
+    }
+    bnd = (unsigned long long)wave_sum64((long long)bnd);
+    if (c.lane == 0) {
+        TaskRef r;
+        r.off = si;
+        r.t = (uint32_t)tc;
+        r.depth = c.depth + 1;
+        r.bound = bnd;
+        r.pad = 0;
+        a.out_refs[s][ri] = r;
+    }
+    c.st.children++;
+    c.st.sum_tw += tc;
+}
+
+// first index in [from, t) where pred holds, else t
+template <class P>
+__device__ __forceinline__ int first_where(Ctx& c, int from, int t, P pred) {
+    for (int base = from; base < t; base += WAVE) {
+        int i = base + c.lane;
+        uint64_t b = ballot(i < t && pred(i));
+        if (b) return base + __builtin_ctzll(b);
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// one task record, main.cu:1060-1181
+// ---------------------------------------------------------------------------
+template <bool LOCAL>
+__device__ void process_task(Ctx& c, const TaskRef ref) {
+    const ExpandArgs& a = *c.a;
+    WS& w = c.w;
+    const int t0 = (int)ref.t;
+    c.depth = ref.depth;
+    c.st.tasks++;
+    c.st.sum_t += t0;
+    if (!LOCAL) c.st.large++;
+
+    for (int i = c.lane; i < t0; i += WAVE) {
+        Vrec v = a.in_verts[ref.off + i];
+        w.pid[i] = v.id;
+        w.plab[i] = v.label;
+        w.pin[i] = v.indeg;
+        w.pex[i] = v.exdeg;
+        w.pl2[i] = 0;
+    }
+    wsync();
+    const int nm = uni(first_where(c, 0, t0, [&](int i) { return w.plab[i] != 1; }));
+    const int cov = uni(first_where(c, nm, t0, [&](int i) { return w.plab[i] != 2; })) - nm;
+
+    // position -> matrix index: local position (LOCAL) or root id
+    int* P2M = LOCAL ? nullptr : w.pid;
+    if (LOCAL) {
+        // t0 x t0 sub-matrices of A1 / A2, rows in position order
+        const int W = (t0 + 63) / 64;
+        c.W = W;
+        c.M1 = w.M1;
+        c.M2 = w.M2;
+        const int rw = a.g.rw;
+        for (int wd = 0; wd < W; wd++) {
+            int col = wd * 64 + c.lane;
+            bool cv = col < t0;
+            int cb = cv ? w.pid[col] : 0;
+            int cw = cb >> 6;
+            uint64_t cm = 1ull << (cb & 63);
+            int i = 0;
+            for (; i + 4 <= t0; i += 4) {
+                int r0 = w.pid[i], r1 = w.pid[i + 1], r2 = w.pid[i + 2], r3 = w.pid[i + 3];
+                uint64_t x0 = a.g.A1[(size_t)r0 * rw + cw], x1 = a.g.A1[(size_t)r1 * rw + cw];
+                uint64_t x2 = a.g.A1[(size_t)r2 * rw + cw], x3 = a.g.A1[(size_t)r3 * rw + cw];
+                uint64_t y0 = a.g.A2[(size_t)r0 * rw + cw], y1 = a.g.A2[(size_t)r1 * rw + cw];
+                uint64_t y2 = a.g.A2[(size_t)r2 * rw + cw], y3 = a.g.A2[(size_t)r3 * rw + cw];
+                uint64_t b0 = ballot(cv && (x0 & cm)), b1 = ballot(cv && (x1 & cm));
+                uint64_t b2 = ballot(cv && (x2 & cm)), b3 = ballot(cv && (x3 & cm));
+                uint64_t e0 = ballot(cv && (y0 & cm)), e1 = ballot(cv && (y1 & cm));
+                uint64_t e2 = ballot(cv && (y2 & cm)), e3 = ballot(cv && (y3 & cm));
+                if (c.lane == 0) {
+                    w.M1[(size_t)(i + 0) * W + wd] = b0;
+                    w.M1[(size_t)(i + 1) * W + wd] = b1;
+                    w.M1[(size_t)(i + 2) * W + wd] = b2;
+                    w.M1[(size_t)(i + 3) * W + wd] = b3;
+                    w.M2[(size_t)(i + 0) * W + wd] = e0;
+                    w.M2[(size_t)(i + 1) * W + wd] = e1;
+                    w.M2[(size_t)(i + 2) * W + wd] = e2;
+                    w.M2[(size_t)(i + 3) * W + wd] = e3;
+                }
+            }
+            for (; i < t0; i++) {
+                int r0 = w.pid[i];
+                uint64_t x0 = a.g.A1[(size_t)r0 * rw + cw];
+                uint64_t y0 = a.g.A2[(size_t)r0 * rw + cw];
+                uint64_t b0 = ballot(cv && (x0 & cm)), e0 = ballot(cv && (y0 & cm));
+                if (c.lane == 0) {
+                    w.M1[(size_t)i * W + wd] = b0;
+                    w.M2[(size_t)i * W + wd] = e0;
+                }
+            }
+        }
+        wsync();
+    } else {
+        c.W = a.g.rw;
+        c.M1 = a.g.A1;
+        c.M2 = a.g.A2;
+    }
+#define PM(i) (LOCAL ? (i) : P2M[(i)])
+
+    // ---- lookahead, main.cu:1349-1405 ----
+    int nc = t0 - nm;
+    {
+        int req = md(c, t0);
+        bool f = false;
+        for (int i = c.lane; i < nm; i += WAVE)
+            if (w.pin[i] + w.pex[i] < req) f = true;
+        if (ballot(f) == 0) {
+            clear_mask(c, w.mk0);
+            for (int i = nm + c.lane; i < t0; i += WAVE) set_mask_bit(w.mk0, PM(i));
+            wsync();
+            bool bad = false;
+            for (int i = nm + c.lane; i < t0; i += WAVE) {
+                int l2 = rowpop(c, c.M2, PM(i), w.mk0);
+                w.pl2[i] = l2;
+                if (l2 < nc - 1 || w.pin[i] + w.pex[i] < req) bad = true;
+            }
+            wsync();
+            if (ballot(bad) == 0) {
+                emit_clique(c, w.pid, t0);
+                return;
+            }
+        }
+    }
+
+    // ---- branch loop, main.cu:1094-1180 ----
+    int tp = t0;
+    const int expansions = nc;
+    for (int j = cov; j < expansions; j++) {
+        if (j != cov) {
+            // remove one vertex, main.cu:1408-1462
+            int mdg = get_mindeg(c, nm);
+            nc--;
+            tp--;
+            int rr = PM(tp);
+            bool fail = false;
+            for (int i = c.lane; i < tp; i += WAVE) {
+                if (gbit(c.M1, c.W, rr, PM(i))) {
+                    int ex = w.pex[i] - 1;
+                    w.pex[i] = ex;
+                    if (i < nm && w.pin[i] + ex < mdg) fail = true;
+                }
+            }
+            wsync();
+            if (ballot(fail)) break;
+        }
+        // child record: members, branch vertex, remaining in order
+        int tc = tp, m = nm, cc = nc;
+        for (int i = c.lane; i < tc; i += WAVE) {
+            int src = i < m ? i : (i == m ? tc - 1 : i - 1);
+            w.cid[i] = w.pid[src];
+            w.cpp[i] = PM(src);
+            int lab = w.plab[src];
+            if (cov > 0 && i >= nm + 1 && i <= nm + cov) lab = 0;
+            if (i == m) lab = 1;
+            w.clab[i] = lab;
+            w.cin[i] = w.pin[src];
+            w.cex[i] = w.pex[src];
+            w.cl2[i] = w.pl2[src];
+        }
+        wsync();
+        // add one vertex, main.cu:1465-1523
+        c.st.branches++;
+        const int vpp = PM(tc - 1);
+        m++;
+        cc--;
+        for (int i = c.lane; i < tc; i += WAVE) {
+            if (gbit(c.M1, c.W, vpp, w.cpp[i])) {
+                w.cin[i]++;
+                w.cex[i]--;
+            }
+        }
+        wsync();
+        // diameter pruning, main.cu:1709-1732
+        int nind = 0;
+        for (int base = m; base < tc; base += WAVE) {
+            int i = base + c.lane;
+            bool in = i < tc;
+            bool ok = in && gbit(c.M2, c.W, vpp, w.cpp[i]);
+            uint64_t bo = ballot(ok);
+            if (ok) w.cind[nind + lanes_below(bo)] = w.cin[i];
+            if (in) w.clab[i] = ok ? 0 : -1;
+            nind += __popcll(bo);
+        }
+        wsync();
+        int ub = 0, lb = 0, med = 0;
+        if (degree_pruning(c, tc, cc, m, ub, lb, med, nind, false)) {
+            if (m >= a.g.min_size) check_for_clique(c, m);
+            continue;
+        }
+        int r = critical_pruning(c, tc, cc, m, ub, lb, med);
+        if (r == 2) continue;
+        if (m >= a.g.min_size) check_for_clique(c, m);
+        if (r == 1) continue;
+        if (cc > 0) write_child(c, tc, m);
+        wsync();
+    }
+#undef PM
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void flush_stats(Ctx& c) {
+    const ExpandArgs& a = *c.a;
+    if (c.lane == 0) {
+        atomicAdd(&a.stats->tasks, c.st.tasks);
+        atomicAdd(&a.stats->branches, c.st.branches);
+        atomicAdd(&a.stats->children, c.st.children);
+        atomicAdd(&a.stats->raw_cliques, c.st.raw);
+        atomicAdd(&a.stats->sum_t, c.st.sum_t);
+        atomicAdd(&a.stats->sum_t_written, c.st.sum_tw);
+        atomicAdd(&a.stats->sum_clique_len, c.st.sum_clq);
+        atomicAdd(&a.stats->large_tasks, c.st.large);
+    }
+}
+
+template <bool LOCAL>
+__device__ __forceinline__ void task_loop(Ctx& c) {
+    const ExpandArgs& a = *c.a;
+    const uint32_t nw = gridDim.x;
+    uint32_t i;
+    if (a.static_sched) {
+        i = blockIdx.x;
+    } else {
+        uint32_t x = 0;
+        if (c.lane == 0) x = atomicAdd(a.task_counter, 1u);
+        i = uni((int)__shfl(x, 0, WAVE));
+    }
+    while (i < a.n_in) {
+        TaskRef ref = a.in_refs[i];
+        if (ref.t <= c.cap)
+            process_task<LOCAL>(c, ref);
+        else if (c.lane == 0)
+            atomicOr(a.error, 2);
+        wsync();
+        if (a.static_sched) {
+            i += nw;
+        } else {
+            uint32_t x = 0;
+            if (c.lane == 0) x = atomicAdd(a.task_counter, 1u);
+            i = uni((int)__shfl(x, 0, WAVE));
+        }
+    }
+    flush_stats(c);
+}
+
+template <int TL>
+__global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict__ ap) {
+    const ExpandArgs& a = *ap;
+    constexpr size_t BYTES = ws_bytes(TL, true, (TL + 63) / 64);
+    __shared__ __attribute__((aligned(16))) char lds[BYTES];
+    Ctx c;
+    c.a = ap;
+    c.lane = lane_id();
+    c.w = make_ws(lds, TL, true, (TL + 63) / 64);
+    c.cap = TL;
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0};
+    task_loop<true>(c);
+}
+
+__global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ ap) {
+    const ExpandArgs& a = *ap;
+    Ctx c;
+    c.a = ap;
+    c.lane = lane_id();
+    char* base = a.slab + (size_t)blockIdx.x * a.slab_bytes;
+    c.w = make_ws(base, a.slab_tcap, false, (uint32_t)a.g.rw);
+    c.cap = a.slab_tcap;
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0};
+    task_loop<false>(c);
+}
+
+}  // namespace qcdev
+
+// explicit instantiations exported for the host driver
+template __global__ void qcdev::expand_local<64>(const ExpandArgs*);
+template __global__ void qcdev::expand_local<128>(const ExpandArgs*);
+template __global__ void qcdev::expand_local<256>(const ExpandArgs*);

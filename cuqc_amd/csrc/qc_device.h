@@ -1,0 +1,96 @@
+// qc_device.h -- device-side data layout shared by the HIP sources and the
+// host drivers.
+#pragma once
+
+#include <cstdint>
+
+// A task record in HBM: `t` vertex entries at `off` in a vertex pool.
+// `bound` caps the vertex slots all children of the task can occupy:
+// a child of branch vertex c holds at most min(t, members + 1 + d2(c))
+// vertices, d2(c) = |N2(c) n root|.
+struct TaskRef {
+    uint64_t off;
+    uint32_t t;
+    uint32_t depth;
+    uint64_t bound;
+    uint64_t pad;
+};
+
+// One task vertex (the reference's Vertex, main.cu:71-79, without lvl2adj,
+// which every record carries as 0 and only the parent's lookahead fills).
+// `id` is the root-local vertex id (0 .. R-1).
+struct __attribute__((aligned(16))) Vrec {
+    int32_t id;
+    int32_t label;  // 1 member, 2 covered (root only), 3 cover (root only), 0 candidate
+    int32_t indeg;
+    int32_t exdeg;
+};
+
+// Output stream of raw cliques: `len` original vertex ids at `off`.
+struct CliqueRef {
+    uint64_t off;
+    uint32_t len;
+    uint32_t pad;
+};
+
+// Counters of one output stream (children or cliques).
+struct StreamCtr {
+    unsigned long long items;   // records
+    unsigned long long slots;   // vertex slots
+};
+
+// Per-call statistics accumulated by the kernels.
+struct DevStats {
+    unsigned long long tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, large_tasks;
+};
+
+// Graph tables on the device: root-space 1-hop / 2-hop bit matrices
+// (row a, bit b set iff b in N(a) / N2(a); rows of `rw` 64-bit words),
+// the integer tables and the id translation.
+struct DevGraph {
+    int32_t R;            // root vertices
+    int32_t rw;           // words per root-space row
+    const uint64_t* A1;   // R x rw
+    const uint64_t* A2;   // R x rw
+    const int32_t* mindeg;
+    int32_t mindeg_n;
+    const int32_t* ubq;
+    int32_t ubq_n;
+    int32_t min_size;
+    int32_t pad;
+    const uint32_t* orig; // root-local id -> input vertex id
+    const int32_t* d2r;   // |N2(a) n root| per root vertex
+};
+
+// Arguments of one expansion launch.
+struct ExpandArgs {
+    DevGraph g;
+    const TaskRef* in_refs;
+    const Vrec* in_verts;
+    uint32_t n_in;
+    int32_t static_sched;
+    unsigned int* task_counter;
+
+    // children: tasks with t <= small_max go to stream 0, others to stream 1
+    TaskRef* out_refs[2];
+    Vrec* out_verts[2];
+    StreamCtr* out_ctr;          // [2]
+    unsigned long long out_ref_cap[2];
+    unsigned long long out_vert_cap[2];
+    uint32_t small_max;
+    uint32_t pad0;
+
+    CliqueRef* clq_refs;
+    uint32_t* clq_verts;
+    StreamCtr* clq_ctr;
+    unsigned long long clq_ref_cap;
+    unsigned long long clq_vert_cap;
+
+    // per-wave HBM workspace for tasks above the LDS tile
+    char* slab;
+    unsigned long long slab_bytes;   // per wave
+    uint32_t slab_tcap;              // vertices per slab
+
+    DevStats* stats;
+    int* error;                      // bit 0: output overflow, bit 1: task above slab capacity
+};

@@ -1,0 +1,100 @@
+// qc_internal.h -- library-internal declarations shared by the host drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "qc_device.h"
+#include "qc_host.h"
+
+struct qc_result {
+    // maximal sets, lexicographic order, each ascending
+    std::vector<uint64_t> max_off{0};
+    std::vector<uint32_t> max_v;
+    // raw sets as emitted
+    std::vector<uint64_t> raw_off{0};
+    std::vector<uint32_t> raw_v;
+    qc_stats st{};
+};
+
+namespace qc {
+
+const char* last_error();
+
+#define QC_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            qc::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return QC_EDEVICE;                                                                 \
+        }                                                                                      \
+    } while (0)
+
+// owning device buffer
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t b) {
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+            return QC_ENOMEM;
+        }
+        bytes = b;
+        return QC_OK;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+// Root task and device tables of one mining instance.
+struct Instance {
+    int32_t n = 0;
+    int32_t R = 0;
+    int32_t rw = 0;
+    double gamma = 0;
+    int32_t min_size = 0;
+    std::vector<Vrec> root;           // root task record (root-local ids)
+    std::vector<uint32_t> orig;       // root-local -> input id
+    std::vector<int32_t> mindeg, ubq;
+    std::vector<int32_t> d2r;         // |N2(a) n root|
+    uint64_t root_bound = 0;          // children slot bound of the root task
+    DBuf A1, A2, d_mindeg, d_ubq, d_orig, d_d2r;
+    DevGraph dg{};
+};
+
+// builds the root task and the device tables (root.hip)
+int build_instance(const qc_graph* g, double gamma, int min_size, Instance& inst, hipStream_t s);
+
+// plain / persistent enumeration (driver.hip, persist.hip)
+struct EnumOut {
+    std::vector<uint64_t> raw_off{0};
+    std::vector<uint32_t> raw_v;
+    DevStats ds{};
+    double kernel_ms = 0;
+    uint64_t levels = 0;
+};
+int enumerate_plain(const Instance& inst, const qc_params& p, EnumOut& out, hipStream_t s);
+
+// RemoveNonMax on the device (nonmax.hip)
+int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uint32_t>& v, std::vector<uint64_t>& mo,
+                         std::vector<uint32_t>& mv, hipStream_t s);
+
+}  // namespace qc

@@ -1,0 +1,297 @@
+// root.hip -- root task construction (initialize_tasks, main.cu:846-998) and
+// the device tables the expansion kernels read.
+//
+// Host part: the degree / |N2| filter (main.cu:872-885), the k-core peel
+// (main.cu:889-949), the cover vertex and covered set (main.cu:953-975) and
+// the Q-ordered root record (main.cu:978-995).
+// Device part: for every root vertex, its rows of the root-space 1-hop and
+// 2-hop bit matrices and its full-graph |N2| (the third Q-order key), with
+// the 2-hop relation of adjToSer.cpp:115-156: N(v) u N(N(v)) minus v.
+//
+// Root-local ids are assigned in increasing input-id order, so comparing
+// root-local ids orders vertices exactly like comparing input ids (the last
+// Q-order key, main.cu:2098-2101).
+
+#include <algorithm>
+#include <cstring>
+
+#include "qc_internal.h"
+
+namespace qcdev {
+
+// One wave per root vertex.  vis: per-wave visited bitmap over all n input
+// vertices (zero on entry and on exit).
+__global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                const uint64_t* __restrict__ off2, const uint32_t* __restrict__ adj2,
+                                                const int32_t* __restrict__ loc, const uint32_t* __restrict__ orig,
+                                                int32_t R, int32_t rw, uint64_t* A1, uint64_t* A2, int32_t* n2out,
+                                                unsigned long long* vis_all, uint64_t vis_words, int has_n2) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long* vis = vis_all + (size_t)blockIdx.x * vis_words;
+    for (int a = blockIdx.x; a < R; a += gridDim.x) {
+        const uint32_t v = orig[a];
+        unsigned long long* row1 = (unsigned long long*)(A1 + (size_t)a * rw);
+        unsigned long long* row2 = (unsigned long long*)(A2 + (size_t)a * rw);
+        const uint64_t b0 = off[v], b1 = off[v + 1];
+        for (uint64_t j = b0 + lane; j < b1; j += 64) {
+            int la = loc[adj[j]];
+            if (la >= 0) atomicOr(&row1[la >> 6], 1ull << (la & 63));
+        }
+        if (has_n2) {
+            for (uint64_t j = off2[v] + lane; j < off2[v + 1]; j += 64) {
+                int la = loc[adj2[j]];
+                if (la >= 0) atomicOr(&row2[la >> 6], 1ull << (la & 63));
+            }
+            if (lane == 0) n2out[a] = (int32_t)(off2[v + 1] - off2[v]);
+            __syncthreads();
+            continue;
+        }
+        if (lane == 0) atomicOr(&vis[v >> 6], 1ull << (v & 63));
+        __syncthreads();
+        int cnt = 0;
+        for (uint64_t j = b0 + lane; j < b1; j += 64) {
+            uint32_t u = adj[j];
+            unsigned long long bit = 1ull << (u & 63);
+            unsigned long long old = atomicOr(&vis[u >> 6], bit);
+            if (!(old & bit)) {
+                cnt++;
+                int la = loc[u];
+                if (la >= 0) atomicOr(&row2[la >> 6], 1ull << (la & 63));
+            }
+        }
+        for (uint64_t j = b0; j < b1; j++) {
+            uint32_t u = adj[j];
+            for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) {
+                uint32_t x = adj[k];
+                unsigned long long bit = 1ull << (x & 63);
+                unsigned long long old = atomicOr(&vis[x >> 6], bit);
+                if (!(old & bit)) {
+                    cnt++;
+                    int la = loc[x];
+                    if (la >= 0) atomicOr(&row2[la >> 6], 1ull << (la & 63));
+                }
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+        if (lane == 0) n2out[a] = cnt;
+        __syncthreads();
+        // restore the visited bitmap to zero
+        if (lane == 0) vis[v >> 6] = 0;
+        for (uint64_t j = b0 + lane; j < b1; j += 64) vis[adj[j] >> 6] = 0;
+        for (uint64_t j = b0; j < b1; j++) {
+            uint32_t u = adj[j];
+            for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) vis[adj[k] >> 6] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// |N2(a) n root| = popcount of row a of A2
+__global__ __launch_bounds__(64) void root_d2(const uint64_t* __restrict__ A2, int32_t R, int32_t rw, int32_t* d2r) {
+    const int lane = threadIdx.x & 63;
+    for (int a = blockIdx.x; a < R; a += gridDim.x) {
+        int s = 0;
+        for (int w = lane; w < rw; w += 64) s += __popcll(A2[(size_t)a * rw + w]);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) d2r[a] = s;
+    }
+}
+
+}  // namespace qcdev
+
+namespace qc {
+
+namespace {
+
+// |N2(v)| over the full graph with an early exit at `cap`
+long n2_capped(const qc_graph* g, int v, std::vector<int>& mark, long cap) {
+    if (g->has_n2) return (long)(g->off2[v + 1] - g->off2[v]);
+    long cnt = 0;
+    mark[v] = v;
+    for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) {
+        uint32_t u = g->adj[j];
+        if (mark[u] != v) {
+            mark[u] = v;
+            cnt++;
+        }
+        for (uint64_t k = g->off[u]; k < g->off[u + 1]; k++) {
+            uint32_t x = g->adj[k];
+            if (mark[x] != v) {
+                mark[x] = v;
+                cnt++;
+            }
+        }
+        if (cnt >= cap) return cnt;
+    }
+    return cnt;
+}
+
+inline int label_rank(int l) { return l == 1 ? 0 : l == 2 ? 1 : l == 0 ? 2 : l == 3 ? 3 : 4; }
+
+}  // namespace
+
+int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
+    const int n = g->n;
+    I.n = n;
+    I.gamma = gamma;
+    I.min_size = ms;
+    build_tables(gamma, n, ms, I.mindeg, I.ubq);
+    const int thr = I.mindeg[ms];
+
+    // ---- filter (main.cu:872-885) ----
+    std::vector<int32_t> lab(n), exd(n);
+    std::vector<int32_t> rem, rmv;
+    std::vector<int> mark(n, -1);
+    for (int i = 0; i < n; i++) {
+        int deg = (int)(g->off[i + 1] - g->off[i]);
+        exd[i] = deg;
+        bool pass = deg >= thr;
+        if (pass && deg < ms - 1) pass = n2_capped(g, i, mark, ms - 1) >= ms - 1;
+        if (pass && g->has_n2) pass = (long)(g->off2[i + 1] - g->off2[i]) >= ms - 1;
+        lab[i] = pass ? 0 : -1;
+        (pass ? rem : rmv).push_back(i);
+    }
+    // ---- peel (main.cu:889-949) ----
+    int nc = n;
+    while ((int)rem.size() < nc / 2) {
+        nc = (int)rem.size();
+        for (int v : rem) exd[v] = 0;
+        for (int v : rem)
+            for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++)
+                if (lab[g->adj[j]] == 0) exd[g->adj[j]]++;
+        std::vector<int32_t> keep;
+        rmv.clear();
+        for (int v : rem) {
+            if (exd[v] >= thr)
+                keep.push_back(v);
+            else {
+                lab[v] = -1;
+                rmv.push_back(v);
+            }
+        }
+        rem.swap(keep);
+    }
+    for (size_t k = 0; k < rmv.size(); k++) {
+        int v = rmv[k];
+        for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) {
+            uint32_t u = g->adj[j];
+            if (lab[u] == 0 && --exd[u] < thr) {
+                lab[u] = -1;
+                rmv.push_back(u);
+            }
+        }
+    }
+    // ---- root-local ids ----
+    I.orig.clear();
+    std::vector<int32_t> loc(n, -1);
+    for (int i = 0; i < n; i++)
+        if (lab[i] == 0) {
+            loc[i] = (int32_t)I.orig.size();
+            I.orig.push_back((uint32_t)i);
+        }
+    const int R = (int)I.orig.size();
+    I.R = R;
+    I.rw = (R + 63) / 64;
+    I.root.clear();
+    if (R == 0) return QC_OK;
+
+    // ---- device tables ----
+    int rc;
+    const size_t mat_bytes = (size_t)R * I.rw * 8;
+    if ((rc = I.A1.alloc(mat_bytes))) return rc;
+    if ((rc = I.A2.alloc(mat_bytes))) return rc;
+    QC_HIP(hipMemsetAsync(I.A1.p, 0, mat_bytes, s));
+    QC_HIP(hipMemsetAsync(I.A2.p, 0, mat_bytes, s));
+    DBuf d_off, d_adj, d_off2, d_adj2, d_loc, d_n2, d_vis;
+    if ((rc = d_off.alloc(g->off.size() * 8))) return rc;
+    if ((rc = d_adj.alloc(g->adj.size() * 4))) return rc;
+    if ((rc = d_loc.alloc((size_t)n * 4))) return rc;
+    if ((rc = d_n2.alloc((size_t)R * 4))) return rc;
+    if ((rc = I.d_orig.alloc((size_t)R * 4))) return rc;
+    QC_HIP(hipMemcpyAsync(d_off.p, g->off.data(), g->off.size() * 8, hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(d_adj.p, g->adj.data(), g->adj.size() * 4, hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(d_loc.p, loc.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(I.d_orig.p, I.orig.data(), (size_t)R * 4, hipMemcpyHostToDevice, s));
+    if (g->has_n2) {
+        if ((rc = d_off2.alloc(g->off2.size() * 8))) return rc;
+        if ((rc = d_adj2.alloc(g->adj2.size() * 4))) return rc;
+        QC_HIP(hipMemcpyAsync(d_off2.p, g->off2.data(), g->off2.size() * 8, hipMemcpyHostToDevice, s));
+        QC_HIP(hipMemcpyAsync(d_adj2.p, g->adj2.data(), g->adj2.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    int blocks = std::min(R, 2048);
+    uint64_t vis_words = ((uint64_t)n + 63) / 64;
+    if (!g->has_n2) {
+        if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;
+        QC_HIP(hipMemsetAsync(d_vis.p, 0, (size_t)blocks * vis_words * 8, s));
+    }
+    hipLaunchKernelGGL(qcdev::root_rows, dim3(blocks), dim3(64), 0, s, d_off.as<uint64_t>(), d_adj.as<uint32_t>(),
+                       d_off2.as<uint64_t>(), d_adj2.as<uint32_t>(), d_loc.as<int32_t>(), I.d_orig.as<uint32_t>(), R,
+                       I.rw, I.A1.as<uint64_t>(), I.A2.as<uint64_t>(), d_n2.as<int32_t>(),
+                       d_vis.as<unsigned long long>(), vis_words, g->has_n2 ? 1 : 0);
+    QC_HIP(hipGetLastError());
+    if ((rc = I.d_d2r.alloc((size_t)R * 4))) return rc;
+    hipLaunchKernelGGL(qcdev::root_d2, dim3(blocks), dim3(64), 0, s, I.A2.as<uint64_t>(), R, I.rw, I.d_d2r.as<int32_t>());
+    QC_HIP(hipGetLastError());
+    I.d2r.resize(R);
+    QC_HIP(hipMemcpyAsync(I.d2r.data(), I.d_d2r.p, (size_t)R * 4, hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> n2(R);
+    QC_HIP(hipMemcpyAsync(n2.data(), d_n2.p, (size_t)R * 4, hipMemcpyDeviceToHost, s));
+    if ((rc = I.d_mindeg.alloc(I.mindeg.size() * 4))) return rc;
+    if ((rc = I.d_ubq.alloc(I.ubq.size() * 4))) return rc;
+    QC_HIP(hipMemcpyAsync(I.d_mindeg.p, I.mindeg.data(), I.mindeg.size() * 4, hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(I.d_ubq.p, I.ubq.data(), I.ubq.size() * 4, hipMemcpyHostToDevice, s));
+    QC_HIP(hipStreamSynchronize(s));
+
+    // ---- cover vertex and covered set (main.cu:953-975) ----
+    int maxd = 0, maxi = -1;
+    for (int a = 0; a < R; a++) {
+        int v = (int)I.orig[a];
+        if (exd[v] > maxd) {
+            maxd = exd[v];
+            maxi = a;
+        }
+    }
+    struct RV {
+        int32_t id, label, exdeg, l2;
+    };
+    std::vector<RV> rv(R);
+    for (int a = 0; a < R; a++) rv[a] = {a, 0, exd[I.orig[a]], n2[a]};
+    if (maxi >= 0) {
+        rv[maxi].label = 3;
+        int v = (int)I.orig[maxi];
+        for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) {
+            int la = loc[g->adj[j]];
+            if (la >= 0 && rv[la].label == 0) rv[la].label = 2;
+        }
+    }
+    // ---- Q order (main.cu:2059-2104); indeg is 0 for every root vertex ----
+    std::sort(rv.begin(), rv.end(), [](const RV& x, const RV& y) {
+        int rx = label_rank(x.label), ry = label_rank(y.label);
+        if (rx != ry) return rx < ry;
+        if (x.exdeg != y.exdeg) return x.exdeg > y.exdeg;
+        if (x.l2 != y.l2) return x.l2 > y.l2;
+        return x.id > y.id;
+    });
+    I.root.resize(R);
+    for (int a = 0; a < R; a++) I.root[a] = Vrec{rv[a].id, rv[a].label, 0, rv[a].exdeg};
+
+    I.dg.R = R;
+    I.dg.rw = I.rw;
+    I.dg.A1 = I.A1.as<uint64_t>();
+    I.dg.A2 = I.A2.as<uint64_t>();
+    I.dg.mindeg = I.d_mindeg.as<int32_t>();
+    I.dg.mindeg_n = (int32_t)I.mindeg.size();
+    I.dg.ubq = I.d_ubq.as<int32_t>();
+    I.dg.ubq_n = (int32_t)I.ubq.size();
+    I.dg.min_size = ms;
+    I.dg.orig = I.d_orig.as<uint32_t>();
+    I.dg.d2r = I.d_d2r.as<int32_t>();
+    I.root_bound = 0;
+    for (int a = 0; a < R; a++) {
+        const Vrec& x = I.root[a];
+        if (x.label != 1) I.root_bound += std::min<uint64_t>((uint64_t)R, (uint64_t)I.d2r[x.id] + 1);
+    }
+    return QC_OK;
+}
+
+}  // namespace qc

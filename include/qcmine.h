@@ -1,0 +1,168 @@
+/*
+ * qcmine.h -- C ABI of the MI355X (gfx950) maximal gamma-quasi-clique miner.
+ *
+ * The reference (Mike12041204/cuQC) has no library interface: its boundary is
+ * the command line `cuQC graph_file gamma min_size output_file
+ * scheduling_toggle` (main.cu:416-519), the serialized graph file
+ * (main.cu:96-122, written by adjToSer.cpp:159-200) and the output file
+ * format (main.cu:4641-4651).  Each entry point below names the reference
+ * routine or behaviour it replaces.  The `qcmine` CLI (cuqc_amd/csrc/
+ * qcmine_main.cpp) is built on these calls and keeps the reference's
+ * arguments, messages and return codes.
+ *
+ * Conventions: every call returns a qc_status; on failure
+ * qc_last_error() returns a thread-local message.  The library never exits
+ * the process.  Handles are owned by the library and released with the
+ * matching *_free call.  Plain pointers and sizes only.
+ */
+#ifndef QCMINE_H
+#define QCMINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum qc_status {
+    QC_OK = 0,
+    QC_EINVAL = 1,    /* bad argument (gamma outside [0.5,1], min_size <= 1, ...) */
+    QC_EIO = 2,       /* unreadable, short or corrupt file */
+    QC_ENOMEM = 3,    /* host or device allocation failed */
+    QC_EOVERFLOW = 4, /* a configured capacity was exceeded; no partial output */
+    QC_EDEVICE = 5,   /* HIP runtime error or no usable gfx950 device */
+    QC_EEMPTY = 6     /* zero quasi-cliques found (main.cu:4833, README.md:108) */
+} qc_status;
+
+typedef struct qc_graph qc_graph;
+typedef struct qc_result qc_result;
+
+/* Execution form of the enumeration. */
+typedef enum qc_mode {
+    QC_MODE_PLAIN = 0,      /* level-synchronous kernel, host loop per level
+                               (the structure of search(), main.cu:606-678) */
+    QC_MODE_PERSISTENT = 1  /* one persistent launch, device task queues */
+} qc_mode;
+
+/* Task order inside the persistent kernel. */
+typedef enum qc_order {
+    QC_ORDER_FIFO = 0,      /* plain LIFO task stack */
+    QC_ORDER_COST = 1       /* MLP cost model orders work; output unchanged */
+} qc_order;
+
+typedef struct qc_params {
+    double gamma;            /* main.cu:440-444, in [0.5, 1] */
+    int32_t min_size;        /* main.cu:445-449, > 1 */
+    int32_t scheduling;      /* main.cu:450-453: 0 dynamic, else static */
+    int32_t mode;            /* qc_mode */
+    int32_t order;           /* qc_order */
+    int32_t workgroups;      /* 0 = auto (persistent: CUs x occupancy) */
+    int32_t lds_tile;        /* largest task (vertices) staged in LDS; 0 = auto */
+    int32_t split_depth;     /* cost-ordered splitting depth; 0 = auto */
+    int32_t reserved0;
+    uint64_t task_capacity;  /* vertex slots per task pool; 0 = auto */
+    uint64_t clique_capacity;/* raw-clique vertex slots; 0 = auto */
+    const void* cost_model;  /* qc_model*, required when order == QC_ORDER_COST */
+} qc_params;
+
+typedef struct qc_stats {
+    uint64_t tasks;          /* task records expanded, root included */
+    uint64_t branches;       /* add-one-vertex steps */
+    uint64_t children;       /* child task records written */
+    uint64_t raw_cliques;    /* raw sets emitted before RemoveNonMax */
+    uint64_t max_cliques;    /* maximal sets after RemoveNonMax */
+    uint64_t sum_t;          /* sum of expanded task sizes (vertices) */
+    uint64_t sum_t_written;  /* sum of written child sizes */
+    uint64_t sum_clique_len; /* sum of raw set sizes */
+    uint64_t root_size;      /* vertices in the root task */
+    uint64_t levels;         /* plain mode: host loop iterations */
+    uint64_t large_tasks;    /* tasks above the LDS tile (HBM-resident) */
+    uint64_t reserved[5];
+    double t_init_ms;        /* root construction + device tables */
+    double t_enum_ms;        /* ENUMERATION window (main.cu:551-685) */
+    double t_nonmax_ms;      /* RemoveNonMax */
+    double t_kernel_ms;      /* expansion kernels only (HIP events) */
+} qc_stats;
+
+/* ---- graphs ---------------------------------------------------------- */
+
+/* Replaces CPU_Graph(ifstream&) (main.cu:96-122): reads the 7-record
+ * serialized text file (V, E, L2, N, N offsets, N2, N2 offsets). */
+int qc_graph_load_serialized(const char* path, qc_graph** out);
+
+/* Replaces adjToSer.cpp:74-107: adjacency-list text, line i = neighbours of
+ * vertex i.  N2 is not built here (the miner derives it). */
+int qc_graph_load_adjlist(const char* path, qc_graph** out);
+
+/* Replaces adjToSer.cpp:115-200: writes the serialized text format with the
+ * full-graph N2 lists (sorted, self excluded). */
+int qc_graph_write_serialized(const qc_graph* g, const char* path);
+
+/* Binary CSR file (header "QCG1", n, nnz, offsets u64, adjacency u32); the
+ * format for graphs whose text / N2 form is impractical. */
+int qc_graph_load_binary(const char* path, qc_graph** out);
+int qc_graph_write_binary(const qc_graph* g, const char* path);
+
+/* Builds a graph from symmetric CSR (copied; lists are sorted, self loops and
+ * duplicates rejected with QC_EINVAL). */
+int qc_graph_from_csr(int32_t n, const uint64_t* offsets, const uint32_t* adj, qc_graph** out);
+
+/* Seeded synthetic workload `config` (0..4); config 3 takes `instance` in
+ * [0, 256).  Also returns the workload's gamma and min_size. */
+int qc_graph_generate(int32_t config, uint64_t seed, int32_t instance, qc_graph** out, double* gamma,
+                      int32_t* min_size);
+
+/* Read-only views; pointers stay valid until qc_graph_free. */
+int qc_graph_csr(const qc_graph* g, int32_t* n, const uint64_t** offsets, const uint32_t** adj);
+uint64_t qc_graph_checksum(const qc_graph* g); /* FNV-1a over offsets and adjacency */
+void qc_graph_free(qc_graph* g);
+
+/* ---- mining ---------------------------------------------------------- */
+
+/* Replaces search() + RemoveNonMax() (main.cu:535-692, 4939-4961): builds
+ * the root task, enumerates on the GPU to completion, collects raw sets and
+ * removes non-maximal ones on the device.  QC_EEMPTY when nothing is found
+ * (the result is still returned, with zero sets). */
+int qc_mine(const qc_graph* g, const qc_params* p, qc_result** out);
+
+/* Many independent instances through one persistent launch (no reference
+ * counterpart; each instance's result equals its own qc_mine). */
+int qc_mine_batch(const qc_graph* const* graphs, const qc_params* params, int32_t count, qc_result** out);
+
+/* Maximal sets, each sorted ascending, in lexicographic order
+ * (OutputMaxSet's trie order, main.cu:4881-4937). */
+int qc_result_view(const qc_result* r, uint64_t* count, const uint64_t** offsets, const uint32_t** verts);
+
+/* Raw sets as emitted (vertex order of the emitting step; set order is
+ * unspecified). */
+int qc_result_raw(const qc_result* r, uint64_t* count, const uint64_t** offsets, const uint32_t** verts);
+
+int qc_result_stats(const qc_result* r, qc_stats* s);
+
+/* OutputOneSet format (main.cu:4641-4651): "%d " for the length and each id,
+ * then "\n".  QC_EEMPTY (and no file) when the result holds no sets. */
+int qc_result_write(const qc_result* r, const char* path);
+
+/* temp.txt-compatible raw dump (flush_cliques, main.cu:1234-1246). */
+int qc_result_write_raw(const qc_result* r, const char* path);
+
+void qc_result_free(qc_result* r);
+
+/* ---- RemoveNonMax on its own ----------------------------------------- */
+
+/* Replaces RemoveNonMax(temp, out) (main.cu:4939-4961) for caller-provided
+ * raw sets: device canonicalisation, de-duplication and subset removal. */
+int qc_remove_nonmax(uint64_t count, const uint64_t* offsets, const uint32_t* verts, qc_result** out);
+
+/* ---- misc ------------------------------------------------------------ */
+
+const char* qc_last_error(void);
+const char* qc_version(void);
+/* 1 if a gfx950 device is usable, else 0 (no error is raised). */
+int qc_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QCMINE_H */

@@ -1,0 +1,143 @@
+"""CPU oracle for the qcmine GPU path (test infrastructure only).
+
+ctypes wrapper of liboracle.so, a C++ restatement of the reference's host
+expand/prune path (main.cu:846-2198), its RemoveNonMax (main.cu:4569-4961),
+the N2 rule of adjToSer.cpp:115-156 and a brute-force enumerator.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .build import OUT, build
+
+_lib = None
+
+STAT_FIELDS = ["tasks", "branches", "children", "raw_cliques", "sum_t", "sum_t_written", "sum_clique_len", "w_adj",
+               "lookahead_hits", "stale_oob", "stale_hits", "max_t", "max_depth", "root_size", "init_w_adj"]
+
+
+class _Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in STAT_FIELDS]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = build()
+        L = C.CDLL(str(path))
+        P = C.c_void_p
+        L.orc_mine.restype = P
+        L.orc_mine.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.orc_count.restype = C.c_uint64
+        L.orc_count.argtypes = [P]
+        L.orc_off.restype = C.POINTER(C.c_uint64)
+        L.orc_off.argtypes = [P]
+        L.orc_verts.restype = C.POINTER(C.c_int)
+        L.orc_verts.argtypes = [P]
+        L.orc_get_stats.argtypes = [P, C.POINTER(_Stats)]
+        L.orc_trace_rows.restype = C.c_uint64
+        L.orc_trace_rows.argtypes = [P]
+        L.orc_trace_feats.restype = C.POINTER(C.c_float)
+        L.orc_trace_feats.argtypes = [P]
+        L.orc_trace_parent.restype = C.POINTER(C.c_int64)
+        L.orc_trace_parent.argtypes = [P]
+        L.orc_free.argtypes = [P]
+        L.orc_nonmax.restype = P
+        L.orc_nonmax.argtypes = [C.c_uint64, P, P]
+        L.orc_build_n2.restype = C.c_uint64
+        L.orc_build_n2.argtypes = [C.c_int, P, P, P, P]
+        L.orc_brute.restype = P
+        L.orc_brute.argtypes = [C.c_int, P, P, C.c_double, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _take(h) -> list[np.ndarray]:
+    L = lib()
+    k = L.orc_count(h)
+    if k == 0:
+        return []
+    off = np.ctypeslib.as_array(L.orc_off(h), shape=(k + 1,)).copy()
+    v = np.ctypeslib.as_array(L.orc_verts(h), shape=(int(off[-1]),)).copy() if off[-1] else np.zeros(0, np.int32)
+    return [v[off[i]:off[i + 1]].astype(np.int64) for i in range(k)]
+
+
+def _csr(off, adj):
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    return off, adj
+
+
+def mine(off, adj, gamma: float, min_size: int, *, faithful: bool = False, trace: bool = False, off2=None,
+         adj2=None) -> dict:
+    """Run the restated host path to completion; returns raw sets, stats and
+    (optionally) the per-task trace."""
+    L = lib()
+    off, adj = _csr(off, adj)
+    o2 = a2 = None
+    if off2 is not None:
+        off2, adj2 = _csr(off2, adj2)
+        o2, a2 = off2.ctypes.data, adj2.ctypes.data
+    h = L.orc_mine(len(off) - 1, off.ctypes.data, adj.ctypes.data if adj.size else None, o2, a2, float(gamma),
+                   int(min_size), int(faithful), int(trace))
+    try:
+        st = _Stats()
+        L.orc_get_stats(h, C.byref(st))
+        out = {"raw": _take(h), "stats": {n: int(getattr(st, n)) for n in STAT_FIELDS}}
+        if trace:
+            n = L.orc_trace_rows(h)
+            out["trace_feats"] = np.ctypeslib.as_array(L.orc_trace_feats(h), shape=(n, 12)).copy() if n else \
+                np.zeros((0, 12), np.float32)
+            out["trace_parent"] = np.ctypeslib.as_array(L.orc_trace_parent(h), shape=(n,)).copy() if n else \
+                np.zeros(0, np.int64)
+        return out
+    finally:
+        L.orc_free(h)
+
+
+def nonmax(sets) -> list[tuple[int, ...]]:
+    """RemoveNonMax semantics: distinct sets not strictly contained in another,
+    each ascending, lexicographic order."""
+    L = lib()
+    sets = [np.asarray(s, np.int32) for s in sets]
+    off = np.zeros(len(sets) + 1, np.uint64)
+    off[1:] = np.cumsum([len(s) for s in sets]) if sets else []
+    v = np.concatenate(sets).astype(np.int32) if sets else np.zeros(0, np.int32)
+    h = L.orc_nonmax(len(sets), off.ctypes.data, v.ctypes.data if v.size else None)
+    try:
+        return [tuple(int(x) for x in s) for s in _take(h)]
+    finally:
+        L.orc_free(h)
+
+
+def brute(off, adj, gamma: float, min_size: int) -> list[tuple[int, ...]]:
+    """All maximal gamma-quasi-cliques of size >= min_size by exhaustive search (n <= 24)."""
+    L = lib()
+    off, adj = _csr(off, adj)
+    assert len(off) - 1 <= 24
+    h = L.orc_brute(len(off) - 1, off.ctypes.data, adj.ctypes.data if adj.size else None, float(gamma), int(min_size))
+    try:
+        return [tuple(int(x) for x in s) for s in _take(h)]
+    finally:
+        L.orc_free(h)
+
+
+def build_n2(off, adj) -> tuple[np.ndarray, np.ndarray]:
+    """Full-graph N2 lists (adjToSer.cpp:115-156)."""
+    L = lib()
+    off, adj = _csr(off, adj)
+    n = len(off) - 1
+    off2 = np.zeros(n + 1, np.uint64)
+    tot = L.orc_build_n2(n, off.ctypes.data, adj.ctypes.data if adj.size else None, off2.ctypes.data, None)
+    adj2 = np.zeros(max(int(tot), 1), np.int32)
+    L.orc_build_n2(n, off.ctypes.data, adj.ctypes.data if adj.size else None, off2.ctypes.data, adj2.ctypes.data)
+    return off2, adj2[:tot]
+
+
+def canon(sets) -> list[tuple[int, ...]]:
+    """Multiset of sets as a sorted list of ascending tuples."""
+    return sorted(tuple(sorted(int(x) for x in s)) for s in sets)
