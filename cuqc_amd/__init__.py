@@ -44,17 +44,17 @@ class _Params(C.Structure):
     _fields_ = [
         ("gamma", C.c_double), ("min_size", C.c_int32), ("scheduling", C.c_int32), ("mode", C.c_int32),
         ("order", C.c_int32), ("workgroups", C.c_int32), ("lds_tile", C.c_int32), ("split_depth", C.c_int32),
-        ("reserved0", C.c_int32), ("task_capacity", C.c_uint64), ("clique_capacity", C.c_uint64),
-        ("cost_model", C.c_void_p),
+        ("max_depth", C.c_int32), ("task_capacity", C.c_uint64), ("clique_capacity", C.c_uint64),
+        ("cost_model", C.c_void_p), ("root_begin", C.c_int64), ("root_end", C.c_int64),
     ]
 
 
 _STAT_U64 = ["tasks", "branches", "children", "raw_cliques", "max_cliques", "sum_t", "sum_t_written",
-             "sum_clique_len", "root_size", "levels", "large_tasks"]
+             "sum_clique_len", "root_size", "levels", "large_tasks", "beyond", "root_branches"]
 
 
 class _Stats(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("reserved", C.c_uint64 * 5)] + [
+    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("reserved", C.c_uint64 * 3)] + [
         ("t_init_ms", C.c_double), ("t_enum_ms", C.c_double), ("t_nonmax_ms", C.c_double), ("t_kernel_ms", C.c_double)]
 
 
@@ -259,7 +259,7 @@ def _result_from(h: C.c_void_p, rc: int) -> Result:
 
 
 def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-            cost_model=None, split_depth=0) -> _Params:
+            cost_model=None, split_depth=0, window=None) -> _Params:
     p = _Params()
     p.gamma = float(gamma)
     p.min_size = int(min_size)
@@ -272,18 +272,23 @@ def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task
     p.task_capacity = int(task_capacity)
     p.clique_capacity = int(clique_capacity)
     p.cost_model = cost_model
+    rb, re_, md = window if window is not None else (0, -1, -1)
+    p.root_begin, p.root_end, p.max_depth = max(0, int(rb)), max(0, int(re_)), max(0, int(md))
     return p
 
 
 def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode: str = "plain",
          order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
-         clique_capacity: int = 0, cost_model=None, split_depth: int = 0) -> Result:
+         clique_capacity: int = 0, cost_model=None, split_depth: int = 0, window=None) -> Result:
     """Mine maximal gamma-quasi-cliques of size >= min_size (search() + RemoveNonMax()).
 
-    An empty result is returned (status QC_EEMPTY) rather than raised.
+    ``window=(root_begin, root_end, max_depth)`` restricts the search to root
+    branches [root_begin, root_end) and to tasks of depth <= max_depth
+    (-1 = unbounded).  An empty result is returned (status QC_EEMPTY) rather
+    than raised.
     """
     p = _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-                cost_model, split_depth)
+                cost_model, split_depth, window)
     h = C.c_void_p()
     rc = _check(load_library().qc_mine(graph._h, C.byref(p), C.byref(h)), ok=(QC_OK, QC_EEMPTY))
     return _result_from(h, rc)
@@ -291,13 +296,14 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
 
 def mine_batch(graphs: Sequence[Graph], gammas: Sequence[float], min_sizes: Sequence[int], *, scheduling: int = 0,
                order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
-               clique_capacity: int = 0, cost_model=None, split_depth: int = 0) -> list[Result]:
+               clique_capacity: int = 0, cost_model=None, split_depth: int = 0, windows=None) -> list[Result]:
     """Many instances through one persistent launch."""
     n = len(graphs)
     ps = (_Params * n)()
     for i in range(n):
         ps[i] = _params(gammas[i], min_sizes[i], scheduling, "persistent", order, lds_tile, workgroups,
-                        task_capacity, clique_capacity, cost_model, split_depth)
+                        task_capacity, clique_capacity, cost_model, split_depth,
+                        windows[i] if windows is not None else None)
     hs = (C.c_void_p * n)(*[g._h for g in graphs])
     out = (C.c_void_p * n)()
     _check(load_library().qc_mine_batch(hs, ps, n, out))

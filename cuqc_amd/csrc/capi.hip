@@ -95,6 +95,8 @@ int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0,
     r->st.sum_t_written = eo.ds.sum_t_written;
     r->st.sum_clique_len = eo.ds.sum_clique_len;
     r->st.large_tasks = eo.ds.large_tasks;
+    r->st.beyond = eo.ds.beyond;
+    r->st.root_branches = I.root_branches;
     r->st.levels = eo.levels;
     r->st.root_size = (uint64_t)I.R;
     double tn = now_ms();

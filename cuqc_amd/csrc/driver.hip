@@ -138,6 +138,9 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     A.slab_tcap = slab_tcap;
     A.stats = d_stats.as<DevStats>();
     A.error = d_err.as<int>();
+    A.root_b0 = p.root_begin < 0 ? 0 : p.root_begin;
+    A.root_b1 = p.root_end > 0 ? p.root_end : -1;
+    A.max_depth = p.max_depth > 0 ? p.max_depth : -1;
 
     hipEvent_t e0, e1;
     QC_HIP(hipEventCreate(&e0));

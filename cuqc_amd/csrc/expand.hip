@@ -154,7 +154,7 @@ __device__ inline WS make_ws(char* base, uint32_t tcap, bool mat, uint32_t mword
 // per-task state
 // ---------------------------------------------------------------------------
 struct LStats {
-    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large;
+    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large, beyond;
 };
 
 struct Ctx {
@@ -633,6 +633,13 @@ __device__ __forceinline__ void q_key(int lab, int in, int ex, int l2, int id, u
 __device__ void write_child(Ctx& c, int tc, int mem) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
+    if (a.max_depth >= 0 && (int)c.depth + 1 > a.max_depth) {
+        // outside the depth window: counted, never expanded
+        c.st.children++;
+        c.st.sum_tw += tc;
+        c.st.beyond++;
+        return;
+    }
     for (int i = c.lane; i < tc; i += WAVE) q_key(w.clab[i], w.cin[i], w.cex[i], w.cl2[i], w.cid[i], w.khi[i], w.klo[i]);
     wsync();
     sort_keys(c, w.khi, w.klo, tc);
@@ -799,7 +806,9 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
     // ---- branch loop, main.cu:1094-1180 ----
     int tp = t0;
     const int expansions = nc;
+    const bool root = ref.depth == 0;
     for (int j = cov; j < expansions; j++) {
+        if (root && a.root_b1 >= 0 && j - cov >= a.root_b1) break;
         if (j != cov) {
             // remove one vertex, main.cu:1408-1462
             int mdg = get_mindeg(c, nm);
@@ -817,6 +826,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             wsync();
             if (ballot(fail)) break;
         }
+        if (root && j - cov < a.root_b0) continue;
         // child record: members, branch vertex, remaining in order
         int tc = tp, m = nm, cc = nc;
         for (int i = c.lane; i < tc; i += WAVE) {
@@ -885,6 +895,7 @@ __device__ __forceinline__ void flush_stats(Ctx& c) {
         atomicAdd(&a.stats->sum_t_written, c.st.sum_tw);
         atomicAdd(&a.stats->sum_clique_len, c.st.sum_clq);
         atomicAdd(&a.stats->large_tasks, c.st.large);
+        atomicAdd(&a.stats->beyond, c.st.beyond);
     }
 }
 
@@ -928,7 +939,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     c.lane = lane_id();
     c.w = make_ws(lds, TL, true, (TL + 63) / 64);
     c.cap = TL;
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0};
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     task_loop<true>(c);
 }
 
@@ -940,7 +951,7 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     char* base = a.slab + (size_t)blockIdx.x * a.slab_bytes;
     c.w = make_ws(base, a.slab_tcap, false, (uint32_t)a.g.rw);
     c.cap = a.slab_tcap;
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0};
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     task_loop<false>(c);
 }
 

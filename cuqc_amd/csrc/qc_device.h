@@ -41,7 +41,8 @@ struct StreamCtr {
 
 // Per-call statistics accumulated by the kernels.
 struct DevStats {
-    unsigned long long tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, large_tasks;
+    unsigned long long tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, large_tasks,
+        beyond;
 };
 
 // Graph tables on the device: root-space 1-hop / 2-hop bit matrices
@@ -90,6 +91,11 @@ struct ExpandArgs {
     char* slab;
     unsigned long long slab_bytes;   // per wave
     uint32_t slab_tcap;              // vertices per slab
+
+    // search window (see qc_params)
+    long long root_b0, root_b1;
+    int32_t max_depth;
+    int32_t pad1;
 
     DevStats* stats;
     int* error;                      // bit 0: output overflow, bit 1: task above slab capacity

@@ -76,6 +76,7 @@ struct Instance {
     std::vector<int32_t> mindeg, ubq;
     std::vector<int32_t> d2r;         // |N2(a) n root|
     uint64_t root_bound = 0;          // children slot bound of the root task
+    uint64_t root_branches = 0;       // branches of the root task
     DBuf A1, A2, d_mindeg, d_ubq, d_orig, d_d2r;
     DevGraph dg{};
 };

@@ -287,6 +287,9 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     I.dg.orig = I.d_orig.as<uint32_t>();
     I.dg.d2r = I.d_d2r.as<int32_t>();
     I.root_bound = 0;
+    I.root_branches = 0;
+    for (int a = 0; a < R; a++)
+        if (I.root[a].label != 2) I.root_branches++;
     for (int a = 0; a < R; a++) {
         const Vrec& x = I.root[a];
         if (x.label != 1) I.root_bound += std::min<uint64_t>((uint64_t)R, (uint64_t)I.d2r[x.id] + 1);

@@ -59,10 +59,12 @@ typedef struct qc_params {
     int32_t workgroups;      /* 0 = auto (persistent: CUs x occupancy) */
     int32_t lds_tile;        /* largest task (vertices) staged in LDS; 0 = auto */
     int32_t split_depth;     /* cost-ordered splitting depth; 0 = auto */
-    int32_t reserved0;
+    int32_t max_depth;       /* search window: expand tasks of depth <= max_depth; <= 0: all */
     uint64_t task_capacity;  /* vertex slots per task pool; 0 = auto */
     uint64_t clique_capacity;/* raw-clique vertex slots; 0 = auto */
     const void* cost_model;  /* qc_model*, required when order == QC_ORDER_COST */
+    int64_t root_begin;      /* search window: root branches [root_begin, root_end) in */
+    int64_t root_end;        /*   the reference's branch order (main.cu:1094); end <= 0: all */
 } qc_params;
 
 typedef struct qc_stats {
@@ -77,7 +79,9 @@ typedef struct qc_stats {
     uint64_t root_size;      /* vertices in the root task */
     uint64_t levels;         /* plain mode: host loop iterations */
     uint64_t large_tasks;    /* tasks above the LDS tile (HBM-resident) */
-    uint64_t reserved[5];
+    uint64_t beyond;         /* child records past the depth window (not expanded) */
+    uint64_t root_branches;  /* branches of the root task (window range) */
+    uint64_t reserved[3];
     double t_init_ms;        /* root construction + device tables */
     double t_enum_ms;        /* ENUMERATION window (main.cu:551-685) */
     double t_nonmax_ms;      /* RemoveNonMax */

@@ -17,7 +17,7 @@ from .build import OUT, build
 _lib = None
 
 STAT_FIELDS = ["tasks", "branches", "children", "raw_cliques", "sum_t", "sum_t_written", "sum_clique_len", "w_adj",
-               "lookahead_hits", "stale_oob", "stale_hits", "max_t", "max_depth", "root_size", "init_w_adj"]
+               "lookahead_hits", "stale_oob", "stale_hits", "max_t", "max_depth", "root_size", "init_w_adj", "beyond"]
 
 
 class _Stats(C.Structure):
@@ -32,6 +32,9 @@ def lib() -> C.CDLL:
         P = C.c_void_p
         L.orc_mine.restype = P
         L.orc_mine.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.orc_mine_window.restype = P
+        L.orc_mine_window.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.c_long,
+                                      C.c_long, C.c_int]
         L.orc_count.restype = C.c_uint64
         L.orc_count.argtypes = [P]
         L.orc_off.restype = C.POINTER(C.c_uint64)
@@ -73,17 +76,19 @@ def _csr(off, adj):
 
 
 def mine(off, adj, gamma: float, min_size: int, *, faithful: bool = False, trace: bool = False, off2=None,
-         adj2=None) -> dict:
-    """Run the restated host path to completion; returns raw sets, stats and
-    (optionally) the per-task trace."""
+         adj2=None, root_begin: int = 0, root_end: int = -1, max_depth: int = -1) -> dict:
+    """Run the restated host path to completion (or over a search window:
+    root branches [root_begin, root_end) and depth <= max_depth); returns raw
+    sets, stats and (optionally) the per-task trace."""
     L = lib()
     off, adj = _csr(off, adj)
     o2 = a2 = None
     if off2 is not None:
         off2, adj2 = _csr(off2, adj2)
         o2, a2 = off2.ctypes.data, adj2.ctypes.data
-    h = L.orc_mine(len(off) - 1, off.ctypes.data, adj.ctypes.data if adj.size else None, o2, a2, float(gamma),
-                   int(min_size), int(faithful), int(trace))
+    h = L.orc_mine_window(len(off) - 1, off.ctypes.data, adj.ctypes.data if adj.size else None, o2, a2,
+                          float(gamma), int(min_size), int(faithful), int(trace), int(root_begin), int(root_end),
+                          int(max_depth))
     try:
         st = _Stats()
         L.orc_get_stats(h, C.byref(st))

@@ -71,6 +71,7 @@ struct Stats {
     uint64_t max_depth;
     uint64_t root_size;
     uint64_t init_w_adj;     // entries scanned while building the root
+    uint64_t beyond;         // records past the depth window (not expanded)
 };
 
 struct TraceRow {
@@ -109,6 +110,11 @@ struct Oracle {
     std::vector<int> mindeg;
     bool faithful = false;
     bool trace = false;
+    // search window: root branches b in [rb0, rb1) (b = j - covered, the
+    // reference's loop order) and tasks of depth <= max_depth are expanded
+    long rb0 = 0, rb1 = -1;
+    int max_depth = -1;
+    int cur_depth = 0;
 
     std::vector<int> pos;     // vertex_order_map
     std::vector<int> dirty;   // ids set since the last full reset
@@ -477,10 +483,13 @@ struct Oracle {
         if (!faithful) full_reset();
         if (lookahead(rec.data(), t, nm, nc)) return;
         std::vector<Vtx> vs;
+        const bool root = cur_depth == 0;
         for (int j = cov; j < expansions; j++) {
+            if (root && rb1 >= 0 && j - cov >= rb1) break;
             if (j != cov) {
                 if (remove_one_vertex(rec.data(), t, nc, nm)) break;
             }
+            if (root && j - cov < rb0) continue;
             vs.resize(t);
             int m = nm, c = nc, tt = t;
             for (int i = 0; i < m; i++) vs[i] = rec[i];
@@ -709,6 +718,11 @@ struct Oracle {
                 trace_rows.push_back(r);
             }
             if ((uint64_t)it.depth > st.max_depth) st.max_depth = it.depth;
+            if (max_depth >= 0 && it.depth > max_depth) {
+                st.beyond++;
+                continue;
+            }
+            cur_depth = it.depth;
             kids.clear();
             int d = it.depth + 1;
             expand_task(it.rec, [&](std::vector<Vtx>&& c) { kids.push_back({std::move(c), d, my}); });
@@ -730,7 +744,7 @@ extern "C" {
 
 struct orc_stats {
     uint64_t tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, w_adj,
-        lookahead_hits, stale_oob, stale_hits, max_t, max_depth, root_size, init_w_adj;
+        lookahead_hits, stale_oob, stale_hits, max_t, max_depth, root_size, init_w_adj, beyond;
 };
 
 struct orc_result {
@@ -742,8 +756,18 @@ struct orc_result {
 };
 
 // Mine a graph to completion.  off2/adj2 may be NULL (N2 derived from N).
+orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
+                            double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth);
+
 orc_result* orc_mine(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                      double gamma, int min_size, int faithful, int trace) {
+    return orc_mine_window(n, off, adj, off2, adj2, gamma, min_size, faithful, trace, 0, -1, -1);
+}
+
+// Same, restricted to root branches [rb0, rb1) (rb1 < 0: all) and to tasks
+// of depth <= max_depth (max_depth < 0: all).
+orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
+                            double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth) {
     Oracle o;
     o.n = n;
     o.off = off;
@@ -754,6 +778,9 @@ orc_result* orc_mine(int n, const uint64_t* off, const int* adj, const uint64_t*
     o.ms = min_size;
     o.faithful = faithful != 0;
     o.trace = trace != 0;
+    o.rb0 = rb0 < 0 ? 0 : rb0;
+    o.rb1 = rb1;
+    o.max_depth = max_depth;
     o.build_tables();
     o.run();
     orc_result* r = new orc_result;
