@@ -50,11 +50,11 @@ class _Params(C.Structure):
 
 
 _STAT_U64 = ["tasks", "branches", "children", "raw_cliques", "max_cliques", "sum_t", "sum_t_written",
-             "sum_clique_len", "root_size", "levels", "large_tasks", "beyond", "root_branches"]
+             "sum_clique_len", "root_size", "levels", "large_tasks", "beyond", "root_branches", "w_adj"]
 
 
 class _Stats(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("reserved", C.c_uint64 * 3)] + [
+    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("reserved", C.c_uint64 * 2)] + [
         ("t_init_ms", C.c_double), ("t_enum_ms", C.c_double), ("t_nonmax_ms", C.c_double), ("t_kernel_ms", C.c_double)]
 
 

@@ -96,6 +96,7 @@ int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0,
     r->st.sum_clique_len = eo.ds.sum_clique_len;
     r->st.large_tasks = eo.ds.large_tasks;
     r->st.beyond = eo.ds.beyond;
+    r->st.w_adj = eo.ds.w_adj;
     r->st.root_branches = I.root_branches;
     r->st.levels = eo.levels;
     r->st.root_size = (uint64_t)I.R;
@@ -229,6 +230,12 @@ int qc_mine(const qc_graph* g, const qc_params* p, qc_result** out) {
         hipStream_t s;
         QC_HIP(hipStreamCreate(&s));
         std::unique_ptr<qc_result> r(new qc_result);
+        // the ENUMERATION window starts with the graph resident in HBM
+        // (allocate_memory precedes the timer in the reference, main.cu:545-551)
+        if ((rc2 = upload_graph(g, s))) {
+            (void)hipStreamDestroy(s);
+            return rc2;
+        }
         QC_HIP(hipDeviceSynchronize());
         double t0 = now_ms();
         Instance I;
@@ -266,6 +273,7 @@ int qc_mine_batch(const qc_graph* const* graphs, const qc_params* params, int32_
         int rc2 = QC_OK;
         hipStream_t s;
         QC_HIP(hipStreamCreate(&s));
+        for (int i = 0; i < count && rc2 == QC_OK; i++) rc2 = upload_graph(graphs[i], s);
         QC_HIP(hipDeviceSynchronize());
         double t0 = now_ms();
         std::vector<std::unique_ptr<Instance>> insts(count);

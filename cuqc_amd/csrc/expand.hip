@@ -154,7 +154,7 @@ __device__ inline WS make_ws(char* base, uint32_t tcap, bool mat, uint32_t mword
 // per-task state
 // ---------------------------------------------------------------------------
 struct LStats {
-    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large, beyond;
+    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large, beyond, wadj;
 };
 
 struct Ctx {
@@ -205,6 +205,12 @@ __device__ __forceinline__ int rowpop(const Ctx& c, const uint64_t* M, int row, 
     int s = 0;
     for (int w = 0; w < c.W; w++) s += __popcll(r[w] & mask[w]);
     return s;
+}
+
+// W_adj bookkeeping: adjacency entries the reference's list scans visit
+// (restricted to root vertices), summed over ids
+__device__ __forceinline__ void wadj_add(Ctx& c, long long x) {
+    if (c.lane == 0) c.st.wadj += (unsigned long long)x;
 }
 
 __device__ inline void clear_mask(Ctx& c, uint64_t* mk) {
@@ -394,6 +400,14 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
     wsync();
     while (nrem > 0 && nrmv > 0) {
         // degree update, main.cu:1775-1807
+        {
+            const bool rc = nrem < nrmv;
+            const int nl = rc ? nrem : nrmv;
+            const int* L = rc ? w.rem : w.rmv;
+            long long sw = 0;
+            for (int k = c.lane; k < nl; k += WAVE) sw += c.a->g.d1r[w.cid[L[k]]];
+            wadj_add(c, wave_sum64(sw));
+        }
         if (nrem < nrmv) {
             if (map_empty) {
                 for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = 0;
@@ -487,12 +501,15 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
     // mask of critical members
     clear_mask(c, w.mk0);
     bool any = false;
+    long long sw = 0;
     for (int k = c.lane; k < m; k += WAVE) {
         if (w.cin[k] + w.cex[k] == thr && w.cex[k] > 0) {
             set_mask_bit(w.mk0, w.cpp[k]);
             any = true;
+            sw += c.a->g.d1r[w.cid[k]];
         }
     }
+    wadj_add(c, wave_sum64(sw));
     wsync();
     // critical-adjacent candidates
     int ncrit = 0;
@@ -548,7 +565,12 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
     }
     // add the critical-adjacent vertices: degree updates and 2-hop counts
     clear_mask(c, w.mk1);
-    for (int k = c.lane; k < ncrit; k += WAVE) set_mask_bit(w.mk1, w.cpp[m + k]);
+    sw = 0;
+    for (int k = c.lane; k < ncrit; k += WAVE) {
+        set_mask_bit(w.mk1, w.cpp[m + k]);
+        sw += c.a->g.d1r[w.cid[m + k]] + c.a->g.d2r[w.cid[m + k]];
+    }
+    wadj_add(c, wave_sum64(sw));
     wsync();
     for (int i = c.lane; i < tc; i += WAVE) {
         int pp = w.cpp[i];
@@ -787,7 +809,12 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             if (w.pin[i] + w.pex[i] < req) f = true;
         if (ballot(f) == 0) {
             clear_mask(c, w.mk0);
-            for (int i = nm + c.lane; i < t0; i += WAVE) set_mask_bit(w.mk0, PM(i));
+            long long sw = 0;
+            for (int i = nm + c.lane; i < t0; i += WAVE) {
+                set_mask_bit(w.mk0, PM(i));
+                sw += a.g.d2r[w.pid[i]];
+            }
+            wadj_add(c, wave_sum64(sw));
             wsync();
             bool bad = false;
             for (int i = nm + c.lane; i < t0; i += WAVE) {
@@ -816,15 +843,34 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             tp--;
             int rr = PM(tp);
             bool fail = false;
+            int fid = INT32_MAX;   // smallest failing member id
             for (int i = c.lane; i < tp; i += WAVE) {
                 if (gbit(c.M1, c.W, rr, PM(i))) {
                     int ex = w.pex[i] - 1;
                     w.pex[i] = ex;
-                    if (i < nm && w.pin[i] + ex < mdg) fail = true;
+                    if (i < nm && w.pin[i] + ex < mdg) {
+                        fail = true;
+                        fid = min(fid, w.pid[i]);
+                    }
                 }
             }
             wsync();
-            if (ballot(fail)) break;
+            if (ballot(fail)) {
+                // the reference scans N(r) in ascending id order and stops
+                // after the first failing member
+                fid = uni(wave_min(fid));
+                const int rid = w.pid[tp];
+                const uint64_t* row = a.g.A1 + (size_t)rid * a.g.rw;
+                int rank = 0;
+                for (int wd = c.lane; wd <= (fid >> 6); wd += WAVE) {
+                    uint64_t x = row[wd];
+                    if (wd == (fid >> 6)) x &= (1ull << (fid & 63)) - 1ull;
+                    rank += __popcll(x);
+                }
+                wadj_add(c, (long long)wave_sum(rank) + 1);
+                break;
+            }
+            wadj_add(c, a.g.d1r[w.pid[tp]]);
         }
         if (root && j - cov < a.root_b0) continue;
         // child record: members, branch vertex, remaining in order
@@ -845,6 +891,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         // add one vertex, main.cu:1465-1523
         c.st.branches++;
         const int vpp = PM(tc - 1);
+        wadj_add(c, (long long)a.g.d1r[w.pid[tc - 1]] + a.g.d2r[w.pid[tc - 1]]);
         m++;
         cc--;
         for (int i = c.lane; i < tc; i += WAVE) {
@@ -896,6 +943,7 @@ __device__ __forceinline__ void flush_stats(Ctx& c) {
         atomicAdd(&a.stats->sum_clique_len, c.st.sum_clq);
         atomicAdd(&a.stats->large_tasks, c.st.large);
         atomicAdd(&a.stats->beyond, c.st.beyond);
+        atomicAdd(&a.stats->w_adj, c.st.wadj);
     }
 }
 
@@ -939,7 +987,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     c.lane = lane_id();
     c.w = make_ws(lds, TL, true, (TL + 63) / 64);
     c.cap = TL;
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     task_loop<true>(c);
 }
 
@@ -951,7 +999,7 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     char* base = a.slab + (size_t)blockIdx.x * a.slab_bytes;
     c.w = make_ws(base, a.slab_tcap, false, (uint32_t)a.g.rw);
     c.cap = a.slab_tcap;
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     task_loop<false>(c);
 }
 

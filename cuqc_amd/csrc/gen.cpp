@@ -9,8 +9,10 @@
 //      (gamma 0.9), mined with gamma 0.9, min_size 10.
 //   1  8,000 vertices with rank-16 Gaussian features; edge iff
 //      |cos(f_i, f_j)| >= t, t the quantile giving 1,000,000 edges.  Features
-//      are drawn from an anisotropic spectrum s_k = (k+1)^-0.75 so that the
-//      degree distribution is heavy-tailed; gamma 0.9, min_size 30.
+//      follow a gene-module model: 32 modules with Gaussian centres mu_k and
+//      Zipf sizes (share ~ 1/(k+1)) hold 90% of the vertices, a member is
+//      mu_k + 0.3 z, the other 10% are isotropic; this gives the heavy-tailed
+//      degrees of a co-expression network.  gamma 0.9, min_size 30.
 //   2  Chung-Lu, n = 100,000, mean degree 20, exponent 2.1 (weights
 //      w_i ~ (i + i0)^(-1/(beta-1)), capped at sqrt(sum w)), plus 50 planted
 //      0.85-QCs of size U[20, 60]; gamma 0.85, min_size 20.
@@ -221,14 +223,30 @@ void plant(Rng& r, int n, const std::vector<int>& sizes, double g, Edges& e) {
     for (uint64_t kk : have) e.push_back({(uint32_t)(kk >> 32), (uint32_t)(kk & 0xffffffffu)});
 }
 
-// |cos| threshold graph over rank-16 features.
-void cosine_edges(Rng& r, int n, int rank, double alpha, uint64_t target, Edges& e) {
+// |cos| threshold graph over rank-16 features drawn from `nmod` modules
+// (Zipf sizes, member = centre + sigma z) and a share `noise` of isotropic
+// vertices.
+void cosine_edges(Rng& r, int n, int rank, int nmod, double sigma, double noise, uint64_t target, Edges& e) {
     std::vector<float> f((size_t)n * rank);
+    std::vector<double> mu((size_t)nmod * rank);
+    for (auto& x : mu) x = r.normal();
+    std::vector<double> w(nmod);
+    double sw = 0;
+    for (int k = 0; k < nmod; k++) {
+        w[k] = 1.0 / (k + 1.0);
+        sw += w[k];
+    }
+    std::vector<int> lab(n, -1);
+    int in_mod = (int)((1.0 - noise) * n), fill = 0;
+    for (int k = 0; k < nmod; k++) {
+        int sz = (int)(w[k] / sw * in_mod);
+        for (int q = 0; q < sz && fill < n; q++) lab[fill++] = k;
+    }
     for (int i = 0; i < n; i++) {
         double nn = 0;
         double tmp[64];
         for (int k = 0; k < rank; k++) {
-            tmp[k] = r.normal() * std::pow((double)(k + 1), -alpha);
+            tmp[k] = lab[i] >= 0 ? mu[(size_t)lab[i] * rank + k] + sigma * r.normal() : r.normal();
             nn += tmp[k] * tmp[k];
         }
         nn = std::sqrt(nn);
@@ -269,7 +287,7 @@ int generate(int config, uint64_t seed, int instance, qc_graph* g, double* gamma
     } else if (config == 1) {
         Rng r(seed);
         n = 8000;
-        cosine_edges(r, n, 16, 0.75, 1000000, e);
+        cosine_edges(r, n, 16, 32, 0.3, 0.1, 1000000, e);
         *gamma = 0.9;
         *min_size = 30;
     } else if (config == 2) {

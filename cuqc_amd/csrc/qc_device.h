@@ -42,7 +42,7 @@ struct StreamCtr {
 // Per-call statistics accumulated by the kernels.
 struct DevStats {
     unsigned long long tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, large_tasks,
-        beyond;
+        beyond, w_adj;
 };
 
 // Graph tables on the device: root-space 1-hop / 2-hop bit matrices
@@ -61,6 +61,7 @@ struct DevGraph {
     int32_t pad;
     const uint32_t* orig; // root-local id -> input vertex id
     const int32_t* d2r;   // |N2(a) n root| per root vertex
+    const int32_t* d1r;   // |N(a) n root| per root vertex
 };
 
 // Arguments of one expansion launch.

@@ -18,6 +18,16 @@ struct qc_graph {
     bool has_n2 = false;
     std::vector<uint64_t> off2;  // n + 1 when has_n2
     std::vector<uint32_t> adj2;
+    // device-resident copy of the arrays above, made by the first mining call
+    // and reused by later ones (owned; released with the graph)
+    void* dev = nullptr;
+    void (*dev_free)(void*) = nullptr;
+    qc_graph() = default;
+    qc_graph(const qc_graph&) = delete;
+    qc_graph& operator=(const qc_graph&) = delete;
+    ~qc_graph() {
+        if (dev && dev_free) dev_free(dev);
+    }
 };
 
 namespace qc {

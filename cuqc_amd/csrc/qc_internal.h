@@ -77,9 +77,12 @@ struct Instance {
     std::vector<int32_t> d2r;         // |N2(a) n root|
     uint64_t root_bound = 0;          // children slot bound of the root task
     uint64_t root_branches = 0;       // branches of the root task
-    DBuf A1, A2, d_mindeg, d_ubq, d_orig, d_d2r;
+    DBuf A1, A2, d_mindeg, d_ubq, d_orig, d_d2r, d_d1r;
     DevGraph dg{};
 };
+
+// uploads the graph's arrays to HBM once (root.hip)
+int upload_graph(const qc_graph* g, hipStream_t s);
 
 // builds the root task and the device tables (root.hip)
 int build_instance(const qc_graph* g, double gamma, int min_size, Instance& inst, hipStream_t s);

@@ -86,14 +86,24 @@ __global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off
     }
 }
 
-// |N2(a) n root| = popcount of row a of A2
-__global__ __launch_bounds__(64) void root_d2(const uint64_t* __restrict__ A2, int32_t R, int32_t rw, int32_t* d2r) {
+// row popcounts: |N2(a) n root| and |N(a) n root|
+__global__ __launch_bounds__(64) void root_d2(const uint64_t* __restrict__ A2, const uint64_t* __restrict__ A1,
+                                              int32_t R, int32_t rw, int32_t* d2r, int32_t* d1r) {
     const int lane = threadIdx.x & 63;
     for (int a = blockIdx.x; a < R; a += gridDim.x) {
-        int s = 0;
-        for (int w = lane; w < rw; w += 64) s += __popcll(A2[(size_t)a * rw + w]);
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) d2r[a] = s;
+        int s = 0, s1 = 0;
+        for (int w = lane; w < rw; w += 64) {
+            s += __popcll(A2[(size_t)a * rw + w]);
+            s1 += __popcll(A1[(size_t)a * rw + w]);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o, 64);
+            s1 += __shfl_xor(s1, o, 64);
+        }
+        if (lane == 0) {
+            d2r[a] = s;
+            d1r[a] = s1;
+        }
     }
 }
 
@@ -129,6 +139,60 @@ long n2_capped(const qc_graph* g, int v, std::vector<int>& mark, long cap) {
 inline int label_rank(int l) { return l == 1 ? 0 : l == 2 ? 1 : l == 0 ? 2 : l == 3 ? 3 : 4; }
 
 }  // namespace
+
+namespace {
+struct DevCSR {
+    DBuf off, adj, off2, adj2;
+};
+void free_devcsr(void* p) { delete (DevCSR*)p; }
+
+// the graph's arrays in HBM, uploaded once per graph
+int device_csr(const qc_graph* g, DevCSR** out, hipStream_t s) {
+    qc_graph* mg = const_cast<qc_graph*>(g);
+    if (!mg->dev) {
+        DevCSR* d = new DevCSR;
+        int rc;
+        if ((rc = d->off.alloc(g->off.size() * 8)) || (rc = d->adj.alloc(g->adj.size() * 4))) {
+            delete d;
+            return rc;
+        }
+        if (hipMemcpyAsync(d->off.p, g->off.data(), g->off.size() * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(d->adj.p, g->adj.data(), g->adj.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess) {
+            delete d;
+            set_error("graph upload failed");
+            return QC_EDEVICE;
+        }
+        if (g->has_n2) {
+            if ((rc = d->off2.alloc(g->off2.size() * 8)) || (rc = d->adj2.alloc(g->adj2.size() * 4))) {
+                delete d;
+                return rc;
+            }
+            if (hipMemcpyAsync(d->off2.p, g->off2.data(), g->off2.size() * 8, hipMemcpyHostToDevice, s) !=
+                    hipSuccess ||
+                hipMemcpyAsync(d->adj2.p, g->adj2.data(), g->adj2.size() * 4, hipMemcpyHostToDevice, s) !=
+                    hipSuccess) {
+                delete d;
+                set_error("graph upload failed");
+                return QC_EDEVICE;
+            }
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) {
+            delete d;
+            set_error("graph upload failed");
+            return QC_EDEVICE;
+        }
+        mg->dev = d;
+        mg->dev_free = free_devcsr;
+    }
+    *out = (DevCSR*)mg->dev;
+    return QC_OK;
+}
+}  // namespace
+
+int upload_graph(const qc_graph* g, hipStream_t s) {
+    DevCSR* d = nullptr;
+    return device_csr(g, &d, s);
+}
 
 int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
     const int n = g->n;
@@ -202,35 +266,29 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     if ((rc = I.A2.alloc(mat_bytes))) return rc;
     QC_HIP(hipMemsetAsync(I.A1.p, 0, mat_bytes, s));
     QC_HIP(hipMemsetAsync(I.A2.p, 0, mat_bytes, s));
-    DBuf d_off, d_adj, d_off2, d_adj2, d_loc, d_n2, d_vis;
-    if ((rc = d_off.alloc(g->off.size() * 8))) return rc;
-    if ((rc = d_adj.alloc(g->adj.size() * 4))) return rc;
+    DevCSR* G = nullptr;
+    if ((rc = device_csr(g, &G, s))) return rc;
+    DBuf d_loc, d_n2, d_vis;
     if ((rc = d_loc.alloc((size_t)n * 4))) return rc;
     if ((rc = d_n2.alloc((size_t)R * 4))) return rc;
     if ((rc = I.d_orig.alloc((size_t)R * 4))) return rc;
-    QC_HIP(hipMemcpyAsync(d_off.p, g->off.data(), g->off.size() * 8, hipMemcpyHostToDevice, s));
-    QC_HIP(hipMemcpyAsync(d_adj.p, g->adj.data(), g->adj.size() * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(d_loc.p, loc.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(I.d_orig.p, I.orig.data(), (size_t)R * 4, hipMemcpyHostToDevice, s));
-    if (g->has_n2) {
-        if ((rc = d_off2.alloc(g->off2.size() * 8))) return rc;
-        if ((rc = d_adj2.alloc(g->adj2.size() * 4))) return rc;
-        QC_HIP(hipMemcpyAsync(d_off2.p, g->off2.data(), g->off2.size() * 8, hipMemcpyHostToDevice, s));
-        QC_HIP(hipMemcpyAsync(d_adj2.p, g->adj2.data(), g->adj2.size() * 4, hipMemcpyHostToDevice, s));
-    }
     int blocks = std::min(R, 2048);
     uint64_t vis_words = ((uint64_t)n + 63) / 64;
     if (!g->has_n2) {
         if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;
         QC_HIP(hipMemsetAsync(d_vis.p, 0, (size_t)blocks * vis_words * 8, s));
     }
-    hipLaunchKernelGGL(qcdev::root_rows, dim3(blocks), dim3(64), 0, s, d_off.as<uint64_t>(), d_adj.as<uint32_t>(),
-                       d_off2.as<uint64_t>(), d_adj2.as<uint32_t>(), d_loc.as<int32_t>(), I.d_orig.as<uint32_t>(), R,
+    hipLaunchKernelGGL(qcdev::root_rows, dim3(blocks), dim3(64), 0, s, G->off.as<uint64_t>(), G->adj.as<uint32_t>(),
+                       G->off2.as<uint64_t>(), G->adj2.as<uint32_t>(), d_loc.as<int32_t>(), I.d_orig.as<uint32_t>(), R,
                        I.rw, I.A1.as<uint64_t>(), I.A2.as<uint64_t>(), d_n2.as<int32_t>(),
                        d_vis.as<unsigned long long>(), vis_words, g->has_n2 ? 1 : 0);
     QC_HIP(hipGetLastError());
     if ((rc = I.d_d2r.alloc((size_t)R * 4))) return rc;
-    hipLaunchKernelGGL(qcdev::root_d2, dim3(blocks), dim3(64), 0, s, I.A2.as<uint64_t>(), R, I.rw, I.d_d2r.as<int32_t>());
+    if ((rc = I.d_d1r.alloc((size_t)R * 4))) return rc;
+    hipLaunchKernelGGL(qcdev::root_d2, dim3(blocks), dim3(64), 0, s, I.A2.as<uint64_t>(), I.A1.as<uint64_t>(), R,
+                       I.rw, I.d_d2r.as<int32_t>(), I.d_d1r.as<int32_t>());
     QC_HIP(hipGetLastError());
     I.d2r.resize(R);
     QC_HIP(hipMemcpyAsync(I.d2r.data(), I.d_d2r.p, (size_t)R * 4, hipMemcpyDeviceToHost, s));
@@ -286,6 +344,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     I.dg.min_size = ms;
     I.dg.orig = I.d_orig.as<uint32_t>();
     I.dg.d2r = I.d_d2r.as<int32_t>();
+    I.dg.d1r = I.d_d1r.as<int32_t>();
     I.root_bound = 0;
     I.root_branches = 0;
     for (int a = 0; a < R; a++)

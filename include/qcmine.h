@@ -81,7 +81,8 @@ typedef struct qc_stats {
     uint64_t large_tasks;    /* tasks above the LDS tile (HBM-resident) */
     uint64_t beyond;         /* child records past the depth window (not expanded) */
     uint64_t root_branches;  /* branches of the root task (window range) */
-    uint64_t reserved[3];
+    uint64_t w_adj;          /* adjacency entries the reference's list scans visit (root-restricted) */
+    uint64_t reserved[2];
     double t_init_ms;        /* root construction + device tables */
     double t_enum_ms;        /* ENUMERATION window (main.cu:551-685) */
     double t_nonmax_ms;      /* RemoveNonMax */

@@ -17,7 +17,8 @@ from .build import OUT, build
 _lib = None
 
 STAT_FIELDS = ["tasks", "branches", "children", "raw_cliques", "sum_t", "sum_t_written", "sum_clique_len", "w_adj",
-               "lookahead_hits", "stale_oob", "stale_hits", "max_t", "max_depth", "root_size", "init_w_adj", "beyond"]
+               "lookahead_hits", "stale_oob", "stale_hits", "max_t", "max_depth", "root_size", "init_w_adj", "beyond",
+               "root_branches"]
 
 
 class _Stats(C.Structure):

@@ -72,6 +72,7 @@ struct Stats {
     uint64_t root_size;
     uint64_t init_w_adj;     // entries scanned while building the root
     uint64_t beyond;         // records past the depth window (not expanded)
+    uint64_t root_branches;  // branches of the root task (window range)
 };
 
 struct TraceRow {
@@ -484,6 +485,7 @@ struct Oracle {
         if (lookahead(rec.data(), t, nm, nc)) return;
         std::vector<Vtx> vs;
         const bool root = cur_depth == 0;
+        if (root) st.root_branches = expansions - cov;
         for (int j = cov; j < expansions; j++) {
             if (root && rb1 >= 0 && j - cov >= rb1) break;
             if (j != cov) {
@@ -744,7 +746,7 @@ extern "C" {
 
 struct orc_stats {
     uint64_t tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, w_adj,
-        lookahead_hits, stale_oob, stale_hits, max_t, max_depth, root_size, init_w_adj, beyond;
+        lookahead_hits, stale_oob, stale_hits, max_t, max_depth, root_size, init_w_adj, beyond, root_branches;
 };
 
 struct orc_result {

@@ -23,6 +23,9 @@ def _compare(res, ref, label):
     assert res.stats["branches"] == ref["stats"]["branches"], label
     assert res.stats["children"] == ref["stats"]["children"], label
     assert res.stats["raw_cliques"] == ref["stats"]["raw_cliques"], label
+    assert res.stats["w_adj"] == ref["stats"]["w_adj"], label
+    assert res.stats["sum_t"] == ref["stats"]["sum_t"], label
+    assert res.stats["sum_t_written"] == ref["stats"]["sum_t_written"], label
     assert oracle.canon(res.raw) == oracle.canon(ref["raw"]), label
     assert res.maximal_sorted_tuples() == oracle.nonmax(ref["raw"]), label
 
