@@ -96,6 +96,7 @@ def load_library() -> C.CDLL:
         "qc_result_write_raw": (C.c_int, [P, C.c_char_p]),
         "qc_result_free": (None, [P]),
         "qc_remove_nonmax": (C.c_int, [C.c_uint64, P, P, PP]),
+        "qc_degree_tables": (C.c_int, [C.c_double, C.c_int32, P, C.c_int32, P]),
         "qc_last_error": (C.c_char_p, []),
         "qc_version": (C.c_char_p, []),
         "qc_device_available": (C.c_int, []),
@@ -308,6 +309,14 @@ def mine_batch(graphs: Sequence[Graph], gammas: Sequence[float], min_sizes: Sequ
     out = (C.c_void_p * n)()
     _check(load_library().qc_mine_batch(hs, ps, n, out))
     return [_result_from(C.c_void_p(out[i]), QC_OK) for i in range(n)]
+
+
+def degree_tables(gamma: float, n: int) -> tuple[np.ndarray, np.ndarray]:
+    """mindeg[0..n) and ubq[0..n) as the library builds them."""
+    md = np.zeros(n, np.int32)
+    uq = np.zeros(n, np.int32)
+    _check(load_library().qc_degree_tables(float(gamma), n, md.ctypes.data, n, uq.ctypes.data))
+    return md, uq
 
 
 def remove_nonmax(sets: Sequence[Sequence[int]]) -> list[np.ndarray]:

@@ -367,6 +367,16 @@ int qc_result_write_raw(const qc_result* r, const char* path) {
 
 void qc_result_free(qc_result* r) { delete r; }
 
+int qc_degree_tables(double gamma, int32_t n_mindeg, int32_t* mindeg, int32_t n_ubq, int32_t* ubq) {
+    if (n_mindeg < 0 || n_ubq < 0 || (n_mindeg && !mindeg) || (n_ubq && !ubq)) return set_error("bad arguments"), QC_EINVAL;
+    std::vector<int32_t> md, uq;
+    int n = std::max(n_mindeg, n_ubq);
+    build_tables(gamma, n, 2, md, uq);
+    for (int i = 0; i < n_mindeg; i++) mindeg[i] = md[i];
+    for (int i = 0; i < n_ubq; i++) ubq[i] = uq[i];
+    return QC_OK;
+}
+
 int qc_remove_nonmax(uint64_t count, const uint64_t* offsets, const uint32_t* verts, qc_result** out) {
     if (!offsets || !out || (!verts && offsets[count] != 0)) return set_error("NULL argument"), QC_EINVAL;
     int rc = device_ok();

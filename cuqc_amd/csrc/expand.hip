@@ -665,19 +665,29 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
     for (int i = c.lane; i < tc; i += WAVE) q_key(w.clab[i], w.cin[i], w.cex[i], w.cl2[i], w.cid[i], w.khi[i], w.klo[i]);
     wsync();
     sort_keys(c, w.khi, w.klo, tc);
+    const bool q = a.queue_mode != 0;
     int s = (uint32_t)tc <= a.small_max ? 0 : 1;
     unsigned long long ri = 0, si = 0;
     if (c.lane == 0) {
-        ri = atomicAdd(&a.out_ctr[s].items, 1ull);
-        si = atomicAdd(&a.out_ctr[s].slots, (unsigned long long)tc);
+        if (q) {
+            ri = atomicAdd(a.q_tail, 1ull);
+            si = atomicAdd(a.pool_top, (unsigned long long)tc);
+        } else {
+            ri = atomicAdd(&a.out_ctr[s].items, 1ull);
+            si = atomicAdd(&a.out_ctr[s].slots, (unsigned long long)tc);
+        }
     }
     ri = __shfl(ri, 0, WAVE);
     si = __shfl(si, 0, WAVE);
-    if (ri >= a.out_ref_cap[s] || si + tc > a.out_vert_cap[s]) {
-        if (c.lane == 0) atomicOr(a.error, 1);
+    if (q ? (ri >= a.q_cap || si + tc > a.pool_cap) : (ri >= a.out_ref_cap[s] || si + tc > a.out_vert_cap[s])) {
+        if (c.lane == 0) {
+            atomicOr(a.error, 1);
+            // the slot stays unpublished: mark it done so no consumer waits
+            if (q && ri < a.q_cap) __hip_atomic_store(&a.q_ready[ri], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
         return;
     }
-    Vrec* dst = a.out_verts[s] + si;
+    Vrec* dst = (q ? a.pool : a.out_verts[s]) + si;
     for (int i = c.lane; i < tc; i += WAVE) {
         uint64_t hi = w.khi[i], lo = w.klo[i];
         uint32_t rank = (uint32_t)(hi >> 61);
@@ -702,7 +712,17 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
         r.depth = c.depth + 1;
         r.bound = bnd;
         r.pad = 0;
-        a.out_refs[s][ri] = r;
+        if (q) {
+            // count the task before it becomes visible, then publish
+            atomicAdd((unsigned long long*)a.pending, 1ull);
+            a.q_refs[ri] = r;
+        } else {
+            a.out_refs[s][ri] = r;
+        }
+    }
+    if (q) {
+        __threadfence();
+        if (c.lane == 0) __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     c.st.children++;
     c.st.sum_tw += tc;
@@ -1003,7 +1023,83 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     task_loop<false>(c);
 }
 
+// ---------------------------------------------------------------------------
+// persistent kernel: every wave pops task refs from the FIFO queue until the
+// pending counter reaches zero; tasks of at most TL vertices run from LDS,
+// larger ones from the wave's HBM slab.
+// ---------------------------------------------------------------------------
+template <int TL>
+__global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __restrict__ ap) {
+    const ExpandArgs& a = *ap;
+    constexpr size_t BYTES = ws_bytes(TL, true, (TL + 63) / 64);
+    __shared__ __attribute__((aligned(16))) char lds[BYTES];
+    Ctx c;
+    c.a = ap;
+    c.lane = lane_id();
+    const WS wl = make_ws(lds, TL, true, (TL + 63) / 64);
+    const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, (uint32_t)a.g.rw);
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int WAIT_LIMIT = 1 << 22;
+    for (;;) {
+        int state = 0;  // 0 idle, 1 got a slot, 2 done
+        unsigned long long h = 0;
+        if (c.lane == 0) {
+            long long p = __hip_atomic_load(a.pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            int err = __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (p <= 0 || (err & 4)) {
+                // search complete, or a slot never got published: every
+                // wave leaves (the host reports the error)
+                state = 2;
+            } else {
+                h = __hip_atomic_load(a.q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned long long t = __hip_atomic_load(a.q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (h < t && h < a.q_cap && atomicCAS(a.q_head, h, h + 1) == h) state = 1;
+            }
+        }
+        state = uni(__shfl(state, 0, WAVE));
+        if (state == 2) break;
+        if (state == 0) {
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+        }
+        h = __shfl(h, 0, WAVE);
+        unsigned int rd = 0;
+        if (c.lane == 0) {
+            // the producer published or is about to publish this slot
+            for (int it = 0; it < WAIT_LIMIT; it++) {
+                rd = __hip_atomic_load(&a.q_ready[h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (rd) break;
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (!rd) atomicOr(a.error, 4);
+        }
+        rd = (unsigned int)uni((int)__shfl(rd, 0, WAVE));
+        if (rd == 1) {
+            const TaskRef ref = a.q_refs[h];
+            if (ref.t <= (uint32_t)TL) {
+                c.w = wl;
+                c.cap = TL;
+                process_task<true>(c, ref);
+            } else if (ref.t <= a.slab_tcap) {
+                c.w = wg;
+                c.cap = a.slab_tcap;
+                process_task<false>(c, ref);
+            } else if (c.lane == 0) {
+                atomicOr(a.error, 2);
+            }
+            wsync();
+            __threadfence();
+            if (c.lane == 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)-1ll);
+        }
+    }
+    flush_stats(c);
+}
+
 }  // namespace qcdev
+
+template __global__ void qcdev::expand_persistent<64>(const ExpandArgs*);
+template __global__ void qcdev::expand_persistent<128>(const ExpandArgs*);
+template __global__ void qcdev::expand_persistent<256>(const ExpandArgs*);
 
 // explicit instantiations exported for the host driver
 template __global__ void qcdev::expand_local<64>(const ExpandArgs*);

@@ -96,8 +96,21 @@ struct ExpandArgs {
     // search window (see qc_params)
     long long root_b0, root_b1;
     int32_t max_depth;
-    int32_t pad1;
+
+    // persistent mode: one FIFO queue of task refs over a bump-allocated
+    // vertex pool; q_ready[i] publishes slot i; `pending` counts tasks pushed
+    // and not yet finished (0 = search complete)
+    int32_t queue_mode;
+    TaskRef* q_refs;
+    unsigned int* q_ready;
+    unsigned long long* q_head;
+    unsigned long long* q_tail;
+    unsigned long long q_cap;
+    Vrec* pool;
+    unsigned long long* pool_top;
+    unsigned long long pool_cap;
+    long long* pending;
 
     DevStats* stats;
-    int* error;                      // bit 0: output overflow, bit 1: task above slab capacity
+    int* error;  // bit 0: output overflow, bit 1: task above slab capacity, bit 2: queue wait limit
 };

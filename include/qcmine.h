@@ -159,6 +159,14 @@ void qc_result_free(qc_result* r);
  * raw sets: device canonicalisation, de-duplication and subset removal. */
 int qc_remove_nonmax(uint64_t count, const uint64_t* offsets, const uint32_t* verts, qc_result** out);
 
+/* ---- tables ---------------------------------------------------------- */
+
+/* Replaces calculate_minimum_degrees (main.cu:526-533) and the floor(d/gamma)
+ * of the LU bounds (main.cu:1905): mindeg[i] = ceil(gamma*(i-1)) for
+ * i < n_mindeg (mindeg[0] = 0) and ubq[d] = floor(d/gamma) for d < n_ubq, in
+ * the reference's double arithmetic. */
+int qc_degree_tables(double gamma, int32_t n_mindeg, int32_t* mindeg, int32_t n_ubq, int32_t* ubq);
+
 /* ---- misc ------------------------------------------------------------ */
 
 const char* qc_last_error(void);

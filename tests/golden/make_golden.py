@@ -1,0 +1,136 @@
+"""Regenerate the golden fixtures in this folder with the CPU oracle.
+
+    python tests/golden/make_golden.py [--windows]
+
+workload0.json      generator checksum, oracle counters and maximal sets of
+                    workload 0 (full search)
+small_vectors.json  seeded small graphs (CSR inline) with task counts and
+                    maximal sets
+map_leak.json       cases where the reference's stale map entries change the
+                    raw output
+windows.json        (--windows) per-workload window results: counters and the
+                    SHA-256 of the maximal-set file, computed by running the
+                    oracle over disjoint root-branch ranges in parallel
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from tests.graphs import SMALL_CASES, planted  # noqa: E402
+
+STATS = ["tasks", "branches", "children", "raw_cliques", "w_adj", "sum_t", "sum_t_written", "sum_clique_len",
+         "beyond"]
+
+
+def digest(sets) -> str:
+    h = hashlib.sha256()
+    for s in sets:
+        h.update((" ".join(str(int(x)) for x in s) + "\n").encode())
+    return h.hexdigest()
+
+
+def workload0():
+    from cuqc_amd import Graph
+    g = Graph.generate(0, 1)
+    off, adj = g.csr()
+    r = oracle.mine(off, adj, g.gamma, g.min_size)
+    mx = oracle.nonmax(r["raw"])
+    return {"checksum": hex(g.checksum()), "gamma": g.gamma, "min_size": g.min_size,
+            "stats": {k: r["stats"][k] for k in STATS}, "maximal": [list(s) for s in mx],
+            "maximal_sha256": digest(mx)}
+
+
+def small_vectors():
+    out = []
+    specs = [(f"planted{i}", 24 + 4 * i, 0.12, [8 + i, 10], 0.9, 0.8, 6) for i in range(6)]
+    specs += [(f"dense{i}", 18 + 2 * i, 0.5, [], 1.0, 0.7, 5) for i in range(6)]
+    for i, (name, n, p, sizes, keep, gamma, ms) in enumerate(specs):
+        off, adj = planted(n, p, sizes, keep, 1000 + i)
+        r = oracle.mine(off, adj, gamma, ms)
+        out.append({"name": name, "off": [int(x) for x in off], "adj": [int(x) for x in adj], "gamma": gamma,
+                    "min_size": ms, "tasks": r["stats"]["tasks"],
+                    "maximal": [list(s) for s in oracle.nonmax(r["raw"])]})
+    return out
+
+
+def map_leak():
+    from cuqc_amd import Graph
+    g = Graph.generate(0, 1)
+    cases = [("config0", *g.csr(), g.gamma, g.min_size)]
+    cases += [(n, *b(), gm, ms) for n, b, gm, ms in SMALL_CASES]
+    diffs = []
+    for name, off, adj, gamma, ms in cases:
+        a = oracle.mine(off, adj, gamma, ms, faithful=False)
+        b = oracle.mine(off, adj, gamma, ms, faithful=True)
+        if oracle.canon(a["raw"]) != oracle.canon(b["raw"]) or a["stats"]["tasks"] != b["stats"]["tasks"]:
+            diffs.append(name)
+    return {"differs": diffs}
+
+
+def _window_part(args):
+    wl, inst, b0, b1 = args
+    from cuqc_amd import workloads
+    w = workloads.WORKLOADS[wl]
+    g = workloads.graph(w, inst)
+    off, adj = g.csr()
+    r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=w.max_depth)
+    return r["stats"], [[int(x) for x in s] for s in r["raw"]]
+
+
+def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dict:
+    from cuqc_amd import workloads
+    w = workloads.WORKLOADS[wl]
+    g = workloads.graph(w, inst)
+    off, adj = g.csr()
+    nb = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=0, root_end=0)["stats"]["root_branches"]
+    b0 = max(0, w.root_begin)
+    b1 = nb if w.root_end < 0 else min(nb, w.root_end)
+    cuts = sorted(set([b0] + [b0 + (b1 - b0) * i // parts for i in range(1, parts)] + [b1]))
+    tasks = [(wl, inst, a, b) for a, b in zip(cuts, cuts[1:])]
+    tot = {k: 0 for k in STATS}
+    raw = []
+    with ProcessPoolExecutor(jobs) as ex:
+        for st, r in ex.map(_window_part, tasks):
+            for k in STATS:
+                tot[k] += st[k] if k != "tasks" else st[k] - 1
+            raw += r
+    tot["tasks"] += 1  # the root task, counted once
+    mx = oracle.nonmax(raw)
+    return {"workload": w.name, "instance": inst, "checksum": hex(g.checksum()), "window": list(w.window),
+            "root_branches": nb, "stats": tot, "maximal": len(mx), "maximal_sha256": digest(mx)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--windows", default="")
+    a = ap.parse_args()
+    with open(os.path.join(HERE, "workload0.json"), "w") as f:
+        json.dump(workload0(), f)
+    with open(os.path.join(HERE, "small_vectors.json"), "w") as f:
+        json.dump(small_vectors(), f)
+    with open(os.path.join(HERE, "map_leak.json"), "w") as f:
+        json.dump(map_leak(), f, indent=1)
+    if a.windows:
+        path = os.path.join(HERE, "windows.json")
+        cur = json.load(open(path)) if os.path.exists(path) else {}
+        for wl in [int(x) for x in a.windows.split(",")]:
+            cur[str(wl)] = window_result(wl)
+            print(json.dumps(cur[str(wl)]), flush=True)
+        with open(path, "w") as f:
+            json.dump(cur, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

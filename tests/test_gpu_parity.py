@@ -56,3 +56,33 @@ def test_small_plain(name, build, gamma, ms):
     for tile in (64, 128):
         res = qc.mine(g, gamma, ms, 0, mode="plain", lds_tile=tile)
         _compare(res, ref, f"{name} tile {tile}")
+
+
+@pytest.mark.parametrize("lds_tile", [64, 128])
+def test_config0_persistent(cfg0, lds_tile):
+    g, ref = cfg0
+    _say(f"[gpu] config0 persistent lds_tile={lds_tile}")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=lds_tile)
+    _compare(res, ref, f"config0 persistent tile {lds_tile}")
+
+
+@pytest.mark.parametrize("name,build,gamma,ms", SMALL_CASES, ids=[c[0] for c in SMALL_CASES])
+def test_small_persistent(name, build, gamma, ms):
+    off, adj = build()
+    _say(f"[gpu] small persistent {name} n={len(off) - 1}")
+    g = qc.Graph.from_csr(off, adj)
+    ref = oracle.mine(off, adj, gamma, ms)
+    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64)
+    _compare(res, ref, f"{name} persistent")
+
+
+@pytest.mark.parametrize("window", [(0, 40, -1), (40, 100, -1), (0, -1, 2), (10, 60, 3)])
+def test_config0_window(cfg0, window):
+    g, _ = cfg0
+    off, adj = g.csr()
+    _say(f"[gpu] config0 window {window}")
+    ref = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=window[0], root_end=window[1], max_depth=window[2])
+    for mode in ("plain", "persistent"):
+        res = qc.mine(g, g.gamma, g.min_size, 0, mode=mode, window=window)
+        _compare(res, ref, f"config0 window {window} {mode}")
+        assert res.stats["beyond"] == ref["stats"]["beyond"]
