@@ -43,7 +43,7 @@ class QCError(RuntimeError):
 class _Params(C.Structure):
     _fields_ = [
         ("gamma", C.c_double), ("min_size", C.c_int32), ("scheduling", C.c_int32), ("mode", C.c_int32),
-        ("order", C.c_int32), ("workgroups", C.c_int32), ("lds_tile", C.c_int32), ("split_depth", C.c_int32),
+        ("order", C.c_int32), ("workgroups", C.c_int32), ("lds_tile", C.c_int32), ("split_chunk", C.c_int32),
         ("max_depth", C.c_int32), ("task_capacity", C.c_uint64), ("clique_capacity", C.c_uint64),
         ("cost_model", C.c_void_p), ("root_begin", C.c_int64), ("root_end", C.c_int64),
     ]
@@ -260,7 +260,7 @@ def _result_from(h: C.c_void_p, rc: int) -> Result:
 
 
 def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-            cost_model=None, split_depth=0, window=None) -> _Params:
+            cost_model=None, split_chunk=0, window=None) -> _Params:
     p = _Params()
     p.gamma = float(gamma)
     p.min_size = int(min_size)
@@ -269,7 +269,7 @@ def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task
     p.order = _ORDERS[order] if isinstance(order, str) else int(order)
     p.lds_tile = int(lds_tile)
     p.workgroups = int(workgroups)
-    p.split_depth = int(split_depth)
+    p.split_chunk = int(split_chunk)
     p.task_capacity = int(task_capacity)
     p.clique_capacity = int(clique_capacity)
     p.cost_model = cost_model
@@ -280,7 +280,7 @@ def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task
 
 def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode: str = "plain",
          order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
-         clique_capacity: int = 0, cost_model=None, split_depth: int = 0, window=None) -> Result:
+         clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, window=None) -> Result:
     """Mine maximal gamma-quasi-cliques of size >= min_size (search() + RemoveNonMax()).
 
     ``window=(root_begin, root_end, max_depth)`` restricts the search to root
@@ -289,7 +289,7 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
     than raised.
     """
     p = _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-                cost_model, split_depth, window)
+                cost_model, split_chunk, window)
     h = C.c_void_p()
     rc = _check(load_library().qc_mine(graph._h, C.byref(p), C.byref(h)), ok=(QC_OK, QC_EEMPTY))
     return _result_from(h, rc)
@@ -297,13 +297,13 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
 
 def mine_batch(graphs: Sequence[Graph], gammas: Sequence[float], min_sizes: Sequence[int], *, scheduling: int = 0,
                order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
-               clique_capacity: int = 0, cost_model=None, split_depth: int = 0, windows=None) -> list[Result]:
+               clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, windows=None) -> list[Result]:
     """Many instances through one persistent launch."""
     n = len(graphs)
     ps = (_Params * n)()
     for i in range(n):
         ps[i] = _params(gammas[i], min_sizes[i], scheduling, "persistent", order, lds_tile, workgroups,
-                        task_capacity, clique_capacity, cost_model, split_depth,
+                        task_capacity, clique_capacity, cost_model, split_chunk,
                         windows[i] if windows is not None else None)
     hs = (C.c_void_p * n)(*[g._h for g in graphs])
     out = (C.c_void_p * n)()

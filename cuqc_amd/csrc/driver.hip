@@ -110,7 +110,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     {
         const int k = I.R <= (int)TL ? 0 : 1;
         QC_HIP(hipMemcpyAsync(st[k].verts.p, I.root.data(), I.root.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
-        TaskRef r{0, (uint32_t)I.R, 0, I.root_bound, 0};
+        TaskRef r{0, (uint32_t)I.R, 0, I.root_bound, 0, 0};
         st[k].refs.push_back(r);
         st[k].top = I.R;
     }

@@ -711,7 +711,8 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
         r.t = (uint32_t)tc;
         r.depth = c.depth + 1;
         r.bound = bnd;
-        r.pad = 0;
+        r.b0 = 0;
+        r.b1 = 0;
         if (q) {
             // count the task before it becomes visible, then publish
             atomicAdd((unsigned long long*)a.pending, 1ull);
@@ -748,17 +749,20 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
     WS& w = c.w;
     const int t0 = (int)ref.t;
     c.depth = ref.depth;
-    c.st.tasks++;
-    c.st.sum_t += t0;
-    if (!LOCAL) c.st.large++;
+    const bool item = ref.b1 > 0;
+    if (!item) {  // a split item's task is counted by its prepare step
+        c.st.tasks++;
+        c.st.sum_t += t0;
+        if (!LOCAL) c.st.large++;
+    }
 
     for (int i = c.lane; i < t0; i += WAVE) {
         Vrec v = a.in_verts[ref.off + i];
         w.pid[i] = v.id;
-        w.plab[i] = v.label;
+        w.plab[i] = item ? (v.label & 15) : v.label;
         w.pin[i] = v.indeg;
         w.pex[i] = v.exdeg;
-        w.pl2[i] = 0;
+        w.pl2[i] = item ? (v.label >> 4) : 0;
     }
     wsync();
     const int nm = uni(first_where(c, 0, t0, [&](int i) { return w.plab[i] != 1; }));
@@ -822,7 +826,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
 
     // ---- lookahead, main.cu:1349-1405 ----
     int nc = t0 - nm;
-    {
+    if (!item) {
         int req = md(c, t0);
         bool f = false;
         for (int i = c.lane; i < nm; i += WAVE)
@@ -851,31 +855,37 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
     }
 
     // ---- branch loop, main.cu:1094-1180 ----
+    // Branch ordinal b = j - covered.  Branch b >= 1 first removes the last
+    // parent vertex (remove_one_vertex); a failing member ends the loop.
     int tp = t0;
     const int expansions = nc;
     const bool root = ref.depth == 0;
-    for (int j = cov; j < expansions; j++) {
-        if (root && a.root_b1 >= 0 && j - cov >= a.root_b1) break;
-        if (j != cov) {
-            // remove one vertex, main.cu:1408-1462
-            int mdg = get_mindeg(c, nm);
-            nc--;
-            tp--;
-            int rr = PM(tp);
-            bool fail = false;
-            int fid = INT32_MAX;   // smallest failing member id
-            for (int i = c.lane; i < tp; i += WAVE) {
-                if (gbit(c.M1, c.W, rr, PM(i))) {
-                    int ex = w.pex[i] - 1;
-                    w.pex[i] = ex;
-                    if (i < nm && w.pin[i] + ex < mdg) {
-                        fail = true;
-                        fid = min(fid, w.pid[i]);
-                    }
+    const int nb = expansions - cov;
+    int bend = nb;
+    if (root && a.root_b1 >= 0 && a.root_b1 < bend) bend = (int)a.root_b1;
+    const int bbeg = root ? (int)min<long long>(a.root_b0, (long long)nb) : 0;
+
+    // remove one vertex, main.cu:1408-1462; returns true on a member failure
+    auto remove_step = [&](bool count) -> bool {
+        int mdg = get_mindeg(c, nm);
+        nc--;
+        tp--;
+        int rr = PM(tp);
+        bool fail = false;
+        int fid = INT32_MAX;  // smallest failing member id
+        for (int i = c.lane; i < tp; i += WAVE) {
+            if (gbit(c.M1, c.W, rr, PM(i))) {
+                int ex = w.pex[i] - 1;
+                w.pex[i] = ex;
+                if (i < nm && w.pin[i] + ex < mdg) {
+                    fail = true;
+                    fid = min(fid, w.pid[i]);
                 }
             }
-            wsync();
-            if (ballot(fail)) {
+        }
+        wsync();
+        if (ballot(fail)) {
+            if (count) {
                 // the reference scans N(r) in ascending id order and stops
                 // after the first failing member
                 fid = uni(wave_min(fid));
@@ -888,11 +898,89 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
                     rank += __popcll(x);
                 }
                 wadj_add(c, (long long)wave_sum(rank) + 1);
+            }
+            return true;
+        }
+        if (count) wadj_add(c, a.g.d1r[w.pid[tp]]);
+        return false;
+    };
+
+    int b_lo = 0, b_hi = nb;
+    bool count_removes = true;
+    if (item) {
+        // split item: apply removals 1..b0 at once (the prepare step found no
+        // failure before its range), then run branches [b0, b1)
+        const int b0 = (int)ref.b0;
+        if (b0 > 0) {
+            clear_mask(c, w.mk0);
+            for (int q = t0 - b0 + c.lane; q < t0; q += WAVE) set_mask_bit(w.mk0, PM(q));
+            wsync();
+            for (int i = c.lane; i < t0 - b0; i += WAVE) w.pex[i] -= rowpop(c, c.M1, PM(i), w.mk0);
+            wsync();
+            tp = t0 - b0;
+            nc -= b0;
+        }
+        b_lo = b0;
+        b_hi = (int)ref.b1;
+        count_removes = false;
+    } else if (!LOCAL && a.queue_mode && a.split_chunk > 0 && (uint32_t)t0 > a.split_min_t &&
+               bend - bbeg > a.split_chunk) {
+        // prepare step: keep the record (with lvl2adj) for the items, find
+        // the first failing removal, count the removals' scans, push items
+        unsigned long long slot = 0;
+        if (c.lane == 0) slot = atomicAdd(a.pool_top, (unsigned long long)t0);
+        slot = __shfl(slot, 0, WAVE);
+        if (slot + t0 > a.pool_cap) {
+            if (c.lane == 0) atomicOr(a.error, 1);
+            return;
+        }
+        for (int i = c.lane; i < t0; i += WAVE) {
+            Vrec v;
+            v.id = w.pid[i];
+            v.label = w.plab[i] | (w.pl2[i] << 4);
+            v.indeg = w.pin[i];
+            v.exdeg = w.pex[i];
+            a.pool[slot + i] = v;
+        }
+        int kb = bend;
+        for (int b = 1; b < bend; b++) {
+            if (remove_step(true)) {
+                kb = b;
                 break;
             }
-            wadj_add(c, a.g.d1r[w.pid[tp]]);
         }
-        if (root && j - cov < a.root_b0) continue;
+        const int hi = min(kb, bend);
+        const int ch = a.split_chunk;
+        __threadfence();
+        for (int b0 = bbeg; b0 < hi; b0 += ch) {
+            if (c.lane == 0) {
+                unsigned long long ri = atomicAdd(a.q_tail, 1ull);
+                if (ri >= a.q_cap) {
+                    atomicOr(a.error, 1);
+                } else {
+                    TaskRef r;
+                    r.off = slot;
+                    r.t = (uint32_t)t0;
+                    r.depth = ref.depth;
+                    r.bound = 0;
+                    r.b0 = (uint32_t)b0;
+                    r.b1 = (uint32_t)min(b0 + ch, hi);
+                    atomicAdd((unsigned long long*)a.pending, 1ull);
+                    a.q_refs[ri] = r;
+                    __threadfence();
+                    __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        return;
+    }
+
+    for (int b = b_lo; b < b_hi; b++) {
+        if (!item && b >= bend) break;
+        if (b > b_lo || (!item && b > 0)) {
+            if (remove_step(count_removes)) break;
+        }
+        if (!item && b < bbeg) continue;
         // child record: members, branch vertex, remaining in order
         int tc = tp, m = nm, cc = nc;
         for (int i = c.lane; i < tc; i += WAVE) {

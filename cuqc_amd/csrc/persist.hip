@@ -78,7 +78,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
 
     // queue: the root at slot 0
     unsigned long long ctr[4] = {0 /*head*/, 1 /*tail*/, (unsigned long long)I.R /*pool top*/, 1 /*pending*/};
-    TaskRef root{0, (uint32_t)I.R, 0, I.root_bound, 0};
+    TaskRef root{0, (uint32_t)I.R, 0, I.root_bound, 0, 0};
     unsigned int one = 1;
     QC_HIP(hipMemsetAsync(d_ready.p, 0, q_cap * 4, s));
     QC_HIP(hipMemcpyAsync(d_pool.p, I.root.data(), I.root.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
@@ -108,6 +108,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     A.root_b1 = p.root_end > 0 ? p.root_end : -1;
     A.max_depth = p.max_depth > 0 ? p.max_depth : -1;
     A.queue_mode = 1;
+    A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 16 : p.split_chunk);
+    A.split_min_t = (uint32_t)TL;
     A.q_refs = d_q.as<TaskRef>();
     A.q_ready = d_ready.as<unsigned int>();
     A.q_head = d_ctr.as<unsigned long long>() + 0;

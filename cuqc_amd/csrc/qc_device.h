@@ -8,12 +8,16 @@
 // `bound` caps the vertex slots all children of the task can occupy:
 // a child of branch vertex c holds at most min(t, members + 1 + d2(c))
 // vertices, d2(c) = |N2(c) n root|.
+// b1 > 0 marks a split item: branches [b0, b1) of a task whose record at
+// `off` was prepared by expand (label | lvl2adj << 4, exdeg before any
+// removal).
 struct TaskRef {
     uint64_t off;
     uint32_t t;
     uint32_t depth;
     uint64_t bound;
-    uint64_t pad;
+    uint32_t b0;
+    uint32_t b1;
 };
 
 // One task vertex (the reference's Vertex, main.cu:71-79, without lvl2adj,
@@ -101,6 +105,8 @@ struct ExpandArgs {
     // vertex pool; q_ready[i] publishes slot i; `pending` counts tasks pushed
     // and not yet finished (0 = search complete)
     int32_t queue_mode;
+    int32_t split_chunk;             // branches per split item (0: no splitting)
+    uint32_t split_min_t;            // split tasks above this size
     TaskRef* q_refs;
     unsigned int* q_ready;
     unsigned long long* q_head;

@@ -86,3 +86,31 @@ def test_config0_window(cfg0, window):
         res = qc.mine(g, g.gamma, g.min_size, 0, mode=mode, window=window)
         _compare(res, ref, f"config0 window {window} {mode}")
         assert res.stats["beyond"] == ref["stats"]["beyond"]
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 16])
+def test_config0_persistent_split(cfg0, chunk):
+    g, ref = cfg0
+    _say(f"[gpu] config0 persistent split chunk={chunk}")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk)
+    _compare(res, ref, f"config0 split {chunk}")
+
+
+@pytest.mark.parametrize("name,build,gamma,ms", SMALL_CASES, ids=[c[0] for c in SMALL_CASES])
+def test_small_persistent_split(name, build, gamma, ms):
+    off, adj = build()
+    _say(f"[gpu] small persistent split {name}")
+    g = qc.Graph.from_csr(off, adj)
+    ref = oracle.mine(off, adj, gamma, ms)
+    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, split_chunk=2)
+    _compare(res, ref, f"{name} split")
+
+
+@pytest.mark.parametrize("window", [(5, 90, -1), (0, -1, 2)])
+def test_config0_window_split(cfg0, window):
+    g, _ = cfg0
+    off, adj = g.csr()
+    _say(f"[gpu] config0 window split {window}")
+    ref = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=window[0], root_end=window[1], max_depth=window[2])
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=4, window=window)
+    _compare(res, ref, f"config0 window {window} split")
