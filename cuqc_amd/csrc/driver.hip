@@ -110,7 +110,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     {
         const int k = I.R <= (int)TL ? 0 : 1;
         QC_HIP(hipMemcpyAsync(st[k].verts.p, I.root.data(), I.root.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
-        TaskRef r{0, (uint32_t)I.R, 0, I.root_bound, 0, 0};
+        TaskRef r{0, (uint32_t)I.R, 0, 0, I.root_bound, 0, 0};
         st[k].refs.push_back(r);
         st[k].top = I.R;
     }
@@ -136,11 +136,12 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     A.slab = d_slab.as<char>();
     A.slab_bytes = slab_b;
     A.slab_tcap = slab_tcap;
+    A.slab_mwords = (uint32_t)I.rw;
     A.stats = d_stats.as<DevStats>();
     A.error = d_err.as<int>();
-    A.root_b0 = p.root_begin < 0 ? 0 : p.root_begin;
-    A.root_b1 = p.root_end > 0 ? p.root_end : -1;
-    A.max_depth = p.max_depth > 0 ? p.max_depth : -1;
+    A.g.root_b0 = p.root_begin < 0 ? 0 : p.root_begin;
+    A.g.root_b1 = p.root_end > 0 ? p.root_end : -1;
+    A.g.max_depth = p.max_depth > 0 ? p.max_depth : -1;
 
     hipEvent_t e0, e1;
     QC_HIP(hipEventCreate(&e0));

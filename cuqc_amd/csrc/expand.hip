@@ -159,6 +159,7 @@ struct LStats {
 
 struct Ctx {
     const ExpandArgs* a;
+    const DevGraph* g;    // tables of the instance being expanded
     WS w;
     const uint64_t* M1;   // matrix used for tests (local or root space)
     const uint64_t* M2;
@@ -167,21 +168,22 @@ struct Ctx {
     LStats st;
     uint32_t depth;
     uint32_t cap;         // largest task the workspace holds
+    uint32_t inst;        // instance of the task being expanded
 };
 
 __device__ __forceinline__ int md(const Ctx& c, int i) {
     // index stays below mindeg_n by construction of the table size
-    return c.a->g.mindeg[i];
+    return c.g->mindeg[i];
 }
 __device__ __forceinline__ int get_mindeg(const Ctx& c, int s) {
-    int ms = c.a->g.min_size;
+    int ms = c.g->min_size;
     return s < ms ? md(c, ms) : md(c, s);
 }
 
 // main.cu:2153-2173
 __device__ __forceinline__ bool cand_valid(const Ctx& c, int in, int ex, int m, int ub, int lb, int med) {
     int tot = in + ex;
-    if (tot < md(c, c.a->g.min_size)) return false;
+    if (tot < md(c, c.g->min_size)) return false;
     if (tot < get_mindeg(c, m + ex + 1)) return false;
     if (tot < med) return false;
     if (in + ub - 1 < md(c, m + lb)) return false;
@@ -191,7 +193,7 @@ __device__ __forceinline__ bool cand_valid(const Ctx& c, int in, int ex, int m, 
 // main.cu:2175-2198
 __device__ __forceinline__ bool vert_ext(const Ctx& c, int in, int ex, int m, int ub, int lb, int med) {
     int tot = in + ex;
-    if (tot < md(c, c.a->g.min_size)) return false;
+    if (tot < md(c, c.g->min_size)) return false;
     if (tot < get_mindeg(c, m + ex)) return false;
     if (tot < med) return false;
     if (ex == 0 && in < get_mindeg(c, m + ex)) return false;
@@ -322,7 +324,7 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med) {
             lb = nc + 1;
             invalid = true;
         }
-        int tq = min_tot < a.g.ubq_n ? a.g.ubq[min_tot] : a.g.ubq[a.g.ubq_n - 1];
+        int tq = min_tot < c.g->ubq_n ? c.g->ubq[min_tot] : c.g->ubq[c.g->ubq_n - 1];
         ub = tq + 1 - m;
         if (ub > nc) ub = nc;
         if (lb < ub) {
@@ -359,10 +361,10 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med) {
         }
     } else {
         ub = nc;
-        int ms = a.g.min_size;
+        int ms = c.g->min_size;
         lb = m < ms ? ms - m : 0;
     }
-    if (m + ub < a.g.min_size) invalid = true;
+    if (m + ub < c.g->min_size) invalid = true;
     if (ub < 0 || ub < lb) invalid = true;
     return invalid;
 }
@@ -405,7 +407,7 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
             const int nl = rc ? nrem : nrmv;
             const int* L = rc ? w.rem : w.rmv;
             long long sw = 0;
-            for (int k = c.lane; k < nl; k += WAVE) sw += c.a->g.d1r[w.cid[L[k]]];
+            for (int k = c.lane; k < nl; k += WAVE) sw += c.g->d1r[w.cid[L[k]]];
             wadj_add(c, wave_sum64(sw));
         }
         if (nrem < nrmv) {
@@ -506,7 +508,7 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
         if (w.cin[k] + w.cex[k] == thr && w.cex[k] > 0) {
             set_mask_bit(w.mk0, w.cpp[k]);
             any = true;
-            sw += c.a->g.d1r[w.cid[k]];
+            sw += c.g->d1r[w.cid[k]];
         }
     }
     wadj_add(c, wave_sum64(sw));
@@ -568,7 +570,7 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
     sw = 0;
     for (int k = c.lane; k < ncrit; k += WAVE) {
         set_mask_bit(w.mk1, w.cpp[m + k]);
-        sw += c.a->g.d1r[w.cid[m + k]] + c.a->g.d2r[w.cid[m + k]];
+        sw += c.g->d1r[w.cid[m + k]] + c.g->d2r[w.cid[m + k]];
     }
     wadj_add(c, wave_sum64(sw));
     wsync();
@@ -620,12 +622,12 @@ __device__ void emit_clique(Ctx& c, const int* ids, int len) {
         if (c.lane == 0) atomicOr(a.error, 1);
         return;
     }
-    for (int k = c.lane; k < len; k += WAVE) a.clq_verts[si + k] = a.g.orig[ids[k]];
+    for (int k = c.lane; k < len; k += WAVE) a.clq_verts[si + k] = c.g->orig[ids[k]];
     if (c.lane == 0) {
         CliqueRef r;
         r.off = si;
         r.len = (uint32_t)len;
-        r.pad = 0;
+        r.inst = c.inst;
         a.clq_refs[ri] = r;
     }
     c.st.raw++;
@@ -655,7 +657,7 @@ __device__ __forceinline__ void q_key(int lab, int in, int ex, int l2, int id, u
 __device__ void write_child(Ctx& c, int tc, int mem) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
-    if (a.max_depth >= 0 && (int)c.depth + 1 > a.max_depth) {
+    if (c.g->max_depth >= 0 && (int)c.depth + 1 > c.g->max_depth) {
         // outside the depth window: counted, never expanded
         c.st.children++;
         c.st.sum_tw += tc;
@@ -701,7 +703,7 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
     // slot bound of this child's own children (see TaskRef)
     unsigned long long bnd = 0;
     for (int i = mem + c.lane; i < tc; i += WAVE) {
-        unsigned long long x = (unsigned long long)a.g.d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
+        unsigned long long x = (unsigned long long)c.g->d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
         bnd += x < (unsigned long long)tc ? x : (unsigned long long)tc;
     }
     bnd = (unsigned long long)wave_sum64((long long)bnd);
@@ -709,7 +711,8 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
         TaskRef r;
         r.off = si;
         r.t = (uint32_t)tc;
-        r.depth = c.depth + 1;
+        r.depth = (uint16_t)(c.depth + 1);
+        r.inst = (uint16_t)c.inst;
         r.bound = bnd;
         r.b0 = 0;
         r.b1 = 0;
@@ -776,7 +779,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         c.W = W;
         c.M1 = w.M1;
         c.M2 = w.M2;
-        const int rw = a.g.rw;
+        const int rw = c.g->rw;
         for (int wd = 0; wd < W; wd++) {
             int col = wd * 64 + c.lane;
             bool cv = col < t0;
@@ -786,10 +789,10 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             int i = 0;
             for (; i + 4 <= t0; i += 4) {
                 int r0 = w.pid[i], r1 = w.pid[i + 1], r2 = w.pid[i + 2], r3 = w.pid[i + 3];
-                uint64_t x0 = a.g.A1[(size_t)r0 * rw + cw], x1 = a.g.A1[(size_t)r1 * rw + cw];
-                uint64_t x2 = a.g.A1[(size_t)r2 * rw + cw], x3 = a.g.A1[(size_t)r3 * rw + cw];
-                uint64_t y0 = a.g.A2[(size_t)r0 * rw + cw], y1 = a.g.A2[(size_t)r1 * rw + cw];
-                uint64_t y2 = a.g.A2[(size_t)r2 * rw + cw], y3 = a.g.A2[(size_t)r3 * rw + cw];
+                uint64_t x0 = c.g->A1[(size_t)r0 * rw + cw], x1 = c.g->A1[(size_t)r1 * rw + cw];
+                uint64_t x2 = c.g->A1[(size_t)r2 * rw + cw], x3 = c.g->A1[(size_t)r3 * rw + cw];
+                uint64_t y0 = c.g->A2[(size_t)r0 * rw + cw], y1 = c.g->A2[(size_t)r1 * rw + cw];
+                uint64_t y2 = c.g->A2[(size_t)r2 * rw + cw], y3 = c.g->A2[(size_t)r3 * rw + cw];
                 uint64_t b0 = ballot(cv && (x0 & cm)), b1 = ballot(cv && (x1 & cm));
                 uint64_t b2 = ballot(cv && (x2 & cm)), b3 = ballot(cv && (x3 & cm));
                 uint64_t e0 = ballot(cv && (y0 & cm)), e1 = ballot(cv && (y1 & cm));
@@ -807,8 +810,8 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             }
             for (; i < t0; i++) {
                 int r0 = w.pid[i];
-                uint64_t x0 = a.g.A1[(size_t)r0 * rw + cw];
-                uint64_t y0 = a.g.A2[(size_t)r0 * rw + cw];
+                uint64_t x0 = c.g->A1[(size_t)r0 * rw + cw];
+                uint64_t y0 = c.g->A2[(size_t)r0 * rw + cw];
                 uint64_t b0 = ballot(cv && (x0 & cm)), e0 = ballot(cv && (y0 & cm));
                 if (c.lane == 0) {
                     w.M1[(size_t)i * W + wd] = b0;
@@ -818,9 +821,9 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         }
         wsync();
     } else {
-        c.W = a.g.rw;
-        c.M1 = a.g.A1;
-        c.M2 = a.g.A2;
+        c.W = c.g->rw;
+        c.M1 = c.g->A1;
+        c.M2 = c.g->A2;
     }
 #define PM(i) (LOCAL ? (i) : P2M[(i)])
 
@@ -836,7 +839,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             long long sw = 0;
             for (int i = nm + c.lane; i < t0; i += WAVE) {
                 set_mask_bit(w.mk0, PM(i));
-                sw += a.g.d2r[w.pid[i]];
+                sw += c.g->d2r[w.pid[i]];
             }
             wadj_add(c, wave_sum64(sw));
             wsync();
@@ -862,8 +865,8 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
     const bool root = ref.depth == 0;
     const int nb = expansions - cov;
     int bend = nb;
-    if (root && a.root_b1 >= 0 && a.root_b1 < bend) bend = (int)a.root_b1;
-    const int bbeg = root ? (int)min<long long>(a.root_b0, (long long)nb) : 0;
+    if (root && c.g->root_b1 >= 0 && c.g->root_b1 < bend) bend = (int)c.g->root_b1;
+    const int bbeg = root ? (int)min<long long>(c.g->root_b0, (long long)nb) : 0;
 
     // remove one vertex, main.cu:1408-1462; returns true on a member failure
     auto remove_step = [&](bool count) -> bool {
@@ -890,7 +893,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
                 // after the first failing member
                 fid = uni(wave_min(fid));
                 const int rid = w.pid[tp];
-                const uint64_t* row = a.g.A1 + (size_t)rid * a.g.rw;
+                const uint64_t* row = c.g->A1 + (size_t)rid * c.g->rw;
                 int rank = 0;
                 for (int wd = c.lane; wd <= (fid >> 6); wd += WAVE) {
                     uint64_t x = row[wd];
@@ -901,7 +904,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             }
             return true;
         }
-        if (count) wadj_add(c, a.g.d1r[w.pid[tp]]);
+        if (count) wadj_add(c, c.g->d1r[w.pid[tp]]);
         return false;
     };
 
@@ -962,6 +965,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
                     r.off = slot;
                     r.t = (uint32_t)t0;
                     r.depth = ref.depth;
+                    r.inst = ref.inst;
                     r.bound = 0;
                     r.b0 = (uint32_t)b0;
                     r.b1 = (uint32_t)min(b0 + ch, hi);
@@ -999,7 +1003,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         // add one vertex, main.cu:1465-1523
         c.st.branches++;
         const int vpp = PM(tc - 1);
-        wadj_add(c, (long long)a.g.d1r[w.pid[tc - 1]] + a.g.d2r[w.pid[tc - 1]]);
+        wadj_add(c, (long long)c.g->d1r[w.pid[tc - 1]] + c.g->d2r[w.pid[tc - 1]]);
         m++;
         cc--;
         for (int i = c.lane; i < tc; i += WAVE) {
@@ -1023,12 +1027,12 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         wsync();
         int ub = 0, lb = 0, med = 0;
         if (degree_pruning(c, tc, cc, m, ub, lb, med, nind, false)) {
-            if (m >= a.g.min_size) check_for_clique(c, m);
+            if (m >= c.g->min_size) check_for_clique(c, m);
             continue;
         }
         int r = critical_pruning(c, tc, cc, m, ub, lb, med);
         if (r == 2) continue;
-        if (m >= a.g.min_size) check_for_clique(c, m);
+        if (m >= c.g->min_size) check_for_clique(c, m);
         if (r == 1) continue;
         if (cc > 0) write_child(c, tc, m);
         wsync();
@@ -1039,21 +1043,22 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void flush_stats(Ctx& c) {
-    const ExpandArgs& a = *c.a;
+__device__ __forceinline__ void flush_stats(Ctx& c, DevStats* d) {
     if (c.lane == 0) {
-        atomicAdd(&a.stats->tasks, c.st.tasks);
-        atomicAdd(&a.stats->branches, c.st.branches);
-        atomicAdd(&a.stats->children, c.st.children);
-        atomicAdd(&a.stats->raw_cliques, c.st.raw);
-        atomicAdd(&a.stats->sum_t, c.st.sum_t);
-        atomicAdd(&a.stats->sum_t_written, c.st.sum_tw);
-        atomicAdd(&a.stats->sum_clique_len, c.st.sum_clq);
-        atomicAdd(&a.stats->large_tasks, c.st.large);
-        atomicAdd(&a.stats->beyond, c.st.beyond);
-        atomicAdd(&a.stats->w_adj, c.st.wadj);
+        if (c.st.tasks) atomicAdd(&d->tasks, c.st.tasks);
+        if (c.st.branches) atomicAdd(&d->branches, c.st.branches);
+        if (c.st.children) atomicAdd(&d->children, c.st.children);
+        if (c.st.raw) atomicAdd(&d->raw_cliques, c.st.raw);
+        if (c.st.sum_t) atomicAdd(&d->sum_t, c.st.sum_t);
+        if (c.st.sum_tw) atomicAdd(&d->sum_t_written, c.st.sum_tw);
+        if (c.st.sum_clq) atomicAdd(&d->sum_clique_len, c.st.sum_clq);
+        if (c.st.large) atomicAdd(&d->large_tasks, c.st.large);
+        if (c.st.beyond) atomicAdd(&d->beyond, c.st.beyond);
+        if (c.st.wadj) atomicAdd(&d->w_adj, c.st.wadj);
     }
+    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 }
+__device__ __forceinline__ void flush_stats(Ctx& c) { flush_stats(c, c.a->stats); }
 
 template <bool LOCAL>
 __device__ __forceinline__ void task_loop(Ctx& c) {
@@ -1092,6 +1097,8 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     __shared__ __attribute__((aligned(16))) char lds[BYTES];
     Ctx c;
     c.a = ap;
+    c.g = &ap->g;
+    c.inst = 0;
     c.lane = lane_id();
     c.w = make_ws(lds, TL, true, (TL + 63) / 64);
     c.cap = TL;
@@ -1103,9 +1110,11 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     const ExpandArgs& a = *ap;
     Ctx c;
     c.a = ap;
+    c.g = &ap->g;
+    c.inst = 0;
     c.lane = lane_id();
     char* base = a.slab + (size_t)blockIdx.x * a.slab_bytes;
-    c.w = make_ws(base, a.slab_tcap, false, (uint32_t)a.g.rw);
+    c.w = make_ws(base, a.slab_tcap, false, a.slab_mwords);
     c.cap = a.slab_tcap;
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     task_loop<false>(c);
@@ -1123,9 +1132,11 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     __shared__ __attribute__((aligned(16))) char lds[BYTES];
     Ctx c;
     c.a = ap;
+    c.g = &ap->g;
+    c.inst = 0;
     c.lane = lane_id();
     const WS wl = make_ws(lds, TL, true, (TL + 63) / 64);
-    const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, (uint32_t)a.g.rw);
+    const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, a.slab_mwords);
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     constexpr int WAIT_LIMIT = 1 << 22;
     for (;;) {
@@ -1164,6 +1175,10 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
         rd = (unsigned int)uni((int)__shfl(rd, 0, WAVE));
         if (rd == 1) {
             const TaskRef ref = a.q_refs[h];
+            if (a.graphs) {
+                c.g = &a.graphs[ref.inst];
+                c.inst = ref.inst;
+            }
             if (ref.t <= (uint32_t)TL) {
                 c.w = wl;
                 c.cap = TL;
@@ -1176,6 +1191,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                 atomicOr(a.error, 2);
             }
             wsync();
+            if (a.inst_stats) flush_stats(c, &a.inst_stats[ref.inst]);
             __threadfence();
             if (c.lane == 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)-1ll);
         }

@@ -1,10 +1,14 @@
-// persist.hip -- persistent-kernel enumeration (one launch per call).
+// persist.hip -- persistent-kernel enumeration: one launch for one or many
+// mining instances.
 //
-// The root record is the first entry of a device FIFO queue; every wave of
-// expand_persistent pops task refs, expands them and pushes the children it
-// writes, until the pending-task counter drops to zero.  There is no host
-// round trip between levels (the reference returns to the host after every
-// level, main.cu:606-678).
+// The roots of all instances seed one device FIFO queue; every wave of
+// expand_persistent pops task refs (tagged with their instance), expands
+// them against that instance's tables and pushes the children it writes,
+// until the pending-task counter drops to zero.  There is no host round trip
+// between levels (the reference returns to the host after every level,
+// main.cu:606-678).  The reset of the queue state and the launch are captured
+// in a hipGraph; the reset is a kernel (no memset nodes), so the graph can be
+// replayed.
 
 #include <algorithm>
 #include <cstring>
@@ -14,6 +18,30 @@
 namespace qcdev {
 template <int TL>
 __global__ void expand_persistent(const ExpandArgs*);
+
+// Queue state for a fresh run: roots in slots [0, nroots), their records at
+// the start of the pool, every other ready flag cleared, counters reset.
+__global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
+                              uint64_t root_slots, TaskRef* q_refs, unsigned int* q_ready, uint64_t q_cap, Vrec* pool,
+                              unsigned long long* ctr, StreamCtr* clq_ctr, DevStats* stats, DevStats* inst_stats,
+                              uint32_t n_inst, int* error) {
+    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = tid; i < q_cap; i += nt) q_ready[i] = i < nroots ? 1u : 0u;
+    for (uint64_t i = tid; i < nroots; i += nt) q_refs[i] = roots[i];
+    for (uint64_t i = tid; i < root_slots; i += nt) pool[i] = root_verts[i];
+    for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (tid == 0) {
+        ctr[0] = 0;            // head
+        ctr[1] = nroots;       // tail
+        ctr[2] = root_slots;   // pool top
+        ctr[3] = nroots;       // pending
+        clq_ctr->items = 0;
+        clq_ctr->slots = 0;
+        *stats = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        *error = 0;
+    }
+}
 }  // namespace qcdev
 
 namespace qc {
@@ -27,24 +55,48 @@ size_t slab_bytes(uint32_t tcap, uint32_t rw) {
     b += (size_t)2 * rw * 8;
     return (b + 255) & ~(size_t)255;
 }
+
+struct RunState {
+    unsigned long long ctr[4];
+    StreamCtr clq;
+    int err;
+};
 }  // namespace
 
 int enumerate_persistent(const Instance* const* insts, const qc_params* params, int count, EnumOut* outs,
                          hipStream_t s) {
-    if (count != 1) {
-        set_error("batched persistent launch: not available in this build");
+    int rc;
+    for (int i = 0; i < count; i++) outs[i] = EnumOut();
+    if (count <= 0) return QC_OK;
+    if (count > 65535) {
+        set_error("at most 65535 instances per batched launch");
         return QC_EINVAL;
     }
-    const Instance& I = *insts[0];
     const qc_params& p = params[0];
-    EnumOut& out = outs[0];
-    out = EnumOut();
-    if (I.R == 0) return QC_OK;
-    int rc;
     const int TL = p.lds_tile == 64 ? 64 : p.lds_tile == 256 ? 256 : 128;
     const void* kern = TL == 64    ? (const void*)qcdev::expand_persistent<64>
                        : TL == 128 ? (const void*)qcdev::expand_persistent<128>
                                    : (const void*)qcdev::expand_persistent<256>;
+
+    // roots and per-instance tables
+    std::vector<TaskRef> roots;
+    std::vector<Vrec> root_verts;
+    std::vector<DevGraph> graphs(count);
+    int maxR = 1, maxrw = 1;
+    for (int i = 0; i < count; i++) {
+        const Instance& I = *insts[i];
+        graphs[i] = I.dg;
+        graphs[i].root_b0 = params[i].root_begin < 0 ? 0 : params[i].root_begin;
+        graphs[i].root_b1 = params[i].root_end > 0 ? params[i].root_end : -1;
+        graphs[i].max_depth = params[i].max_depth > 0 ? params[i].max_depth : -1;
+        if (I.R == 0) continue;
+        maxR = std::max(maxR, I.R);
+        maxrw = std::max(maxrw, I.rw);
+        TaskRef r{root_verts.size(), (uint32_t)I.R, 0, (uint16_t)i, I.root_bound, 0, 0};
+        roots.push_back(r);
+        root_verts.insert(root_verts.end(), I.root.begin(), I.root.end());
+    }
+    if (roots.empty()) return QC_OK;
 
     int dev = 0, n_cu = 0, per_cu = 0;
     QC_HIP(hipGetDevice(&dev));
@@ -56,42 +108,40 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
 
     size_t freeb = 0, totalb = 0;
     QC_HIP(hipMemGetInfo(&freeb, &totalb));
-    const size_t slab_b = slab_bytes((uint32_t)I.R, (uint32_t)I.rw);
+    const size_t slab_b = slab_bytes((uint32_t)maxR, (uint32_t)maxrw);
     const size_t budget = std::min<size_t>(freeb / 2, (size_t)128 << 30);
     if ((size_t)grid * slab_b > budget / 4) grid = (uint32_t)std::max<size_t>(1, budget / 4 / slab_b);
-    // capacities: the FIFO keeps every record it ever held (no reuse), so
-    // they bound the number of tasks / vertex slots / raw cliques per call
-    const uint64_t pool_cap =
-        p.task_capacity ? p.task_capacity : std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec);
-    const uint64_t q_cap = std::min<uint64_t>(pool_cap / 8 + 1024, (uint64_t)1 << 26);
+    // capacities: the FIFO keeps every record it held (no reuse), so they
+    // bound the tasks / vertex slots / raw cliques of one call
+    const uint64_t pool_cap = std::max<uint64_t>(
+        root_verts.size() + 1024,
+        p.task_capacity ? p.task_capacity : std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec));
+    const uint64_t q_cap = std::max<uint64_t>(roots.size() + 1024, std::min<uint64_t>(pool_cap / 8 + 1024, 1ull << 26));
     const uint64_t clq_slots =
         p.clique_capacity ? p.clique_capacity : std::min<uint64_t>(budget / 16 / 4, (uint64_t)1 << 30);
     const uint64_t clq_refs = std::min<uint64_t>(clq_slots / 4 + 1024, (uint64_t)1 << 26);
 
-    DBuf d_pool, d_q, d_ready, d_ctr, d_slab, d_clq_refs, d_clq_verts, d_clq_ctr, d_stats, d_err, d_args;
+    DBuf d_pool, d_q, d_ready, d_ctr, d_slab, d_clq_refs, d_clq_verts, d_clq_ctr, d_stats, d_istats, d_err, d_args,
+        d_graphs, d_roots, d_root_verts;
     if ((rc = d_pool.alloc(pool_cap * sizeof(Vrec))) || (rc = d_q.alloc(q_cap * sizeof(TaskRef))) ||
         (rc = d_ready.alloc(q_cap * 4)) || (rc = d_ctr.alloc(4 * 8)) || (rc = d_slab.alloc((size_t)grid * slab_b)) ||
         (rc = d_clq_refs.alloc(clq_refs * sizeof(CliqueRef))) || (rc = d_clq_verts.alloc(clq_slots * 4)) ||
         (rc = d_clq_ctr.alloc(sizeof(StreamCtr))) || (rc = d_stats.alloc(sizeof(DevStats))) ||
-        (rc = d_err.alloc(sizeof(int))) || (rc = d_args.alloc(sizeof(ExpandArgs))))
+        (rc = d_istats.alloc((size_t)count * sizeof(DevStats))) || (rc = d_err.alloc(sizeof(int))) ||
+        (rc = d_args.alloc(sizeof(ExpandArgs))) || (rc = d_graphs.alloc((size_t)count * sizeof(DevGraph))) ||
+        (rc = d_roots.alloc(roots.size() * sizeof(TaskRef))) ||
+        (rc = d_root_verts.alloc(root_verts.size() * sizeof(Vrec))))
         return rc;
-
-    // queue: the root at slot 0
-    unsigned long long ctr[4] = {0 /*head*/, 1 /*tail*/, (unsigned long long)I.R /*pool top*/, 1 /*pending*/};
-    TaskRef root{0, (uint32_t)I.R, 0, I.root_bound, 0, 0};
-    unsigned int one = 1;
-    QC_HIP(hipMemsetAsync(d_ready.p, 0, q_cap * 4, s));
-    QC_HIP(hipMemcpyAsync(d_pool.p, I.root.data(), I.root.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
-    QC_HIP(hipMemcpyAsync(d_q.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
-    QC_HIP(hipMemcpyAsync(d_ready.p, &one, 4, hipMemcpyHostToDevice, s));
-    QC_HIP(hipMemcpyAsync(d_ctr.p, ctr, sizeof(ctr), hipMemcpyHostToDevice, s));
-    QC_HIP(hipMemsetAsync(d_clq_ctr.p, 0, sizeof(StreamCtr), s));
-    QC_HIP(hipMemsetAsync(d_stats.p, 0, sizeof(DevStats), s));
-    QC_HIP(hipMemsetAsync(d_err.p, 0, sizeof(int), s));
+    QC_HIP(hipMemcpyAsync(d_graphs.p, graphs.data(), (size_t)count * sizeof(DevGraph), hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(d_roots.p, roots.data(), roots.size() * sizeof(TaskRef), hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(d_root_verts.p, root_verts.data(), root_verts.size() * sizeof(Vrec), hipMemcpyHostToDevice,
+                          s));
 
     ExpandArgs A;
     memset(&A, 0, sizeof(A));
-    A.g = I.dg;
+    A.g = graphs[0];
+    A.graphs = d_graphs.as<DevGraph>();
+    A.inst_stats = d_istats.as<DevStats>();
     A.in_verts = d_pool.as<Vrec>();
     A.small_max = (uint32_t)TL;
     A.clq_refs = d_clq_refs.as<CliqueRef>();
@@ -101,12 +151,10 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     A.clq_vert_cap = clq_slots;
     A.slab = d_slab.as<char>();
     A.slab_bytes = slab_b;
-    A.slab_tcap = (uint32_t)I.R;
+    A.slab_tcap = (uint32_t)maxR;
+    A.slab_mwords = (uint32_t)maxrw;
     A.stats = d_stats.as<DevStats>();
     A.error = d_err.as<int>();
-    A.root_b0 = p.root_begin < 0 ? 0 : p.root_begin;
-    A.root_b1 = p.root_end > 0 ? p.root_end : -1;
-    A.max_depth = p.max_depth > 0 ? p.max_depth : -1;
     A.queue_mode = 1;
     A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 16 : p.split_chunk);
     A.split_min_t = (uint32_t)TL;
@@ -120,41 +168,94 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     A.pool = d_pool.as<Vrec>();
     A.pool_cap = pool_cap;
     QC_HIP(hipMemcpyAsync(d_args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
+    QC_HIP(hipStreamSynchronize(s));
+
+    // capture reset + expansion
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const ExpandArgs* ap = d_args.as<ExpandArgs>();
+    void* kargs[] = {(void*)&ap};
+    const uint32_t nroots = (uint32_t)roots.size();
+    QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
+                       d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), d_q.as<TaskRef>(),
+                       d_ready.as<unsigned int>(), q_cap, d_pool.as<Vrec>(), d_ctr.as<unsigned long long>(),
+                       d_clq_ctr.as<StreamCtr>(), d_stats.as<DevStats>(), d_istats.as<DevStats>(),
+                       (uint32_t)count, d_err.as<int>());
+    hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, 0, s);
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_error("graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+        return QC_EDEVICE;
+    }
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        set_error("hipGraphInstantiate failed");
+        return QC_EDEVICE;
+    }
 
     hipEvent_t e0, e1;
     QC_HIP(hipEventCreate(&e0));
     QC_HIP(hipEventCreate(&e1));
-    QC_HIP(hipEventRecord(e0, s));
-    const ExpandArgs* ap = d_args.as<ExpandArgs>();
-    void* kargs[] = {(void*)&ap};
-    QC_HIP(hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, 0, s));
-    QC_HIP(hipEventRecord(e1, s));
-    QC_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    QC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const int replays = std::max(1, p.replays);
+    RunState first{};
+    std::vector<DevStats> first_stats;
+    float first_ms = 0;
+    int status = QC_OK;
+    for (int rep = 0; rep < replays && status == QC_OK; rep++) {
+        QC_HIP(hipEventRecord(e0, s));
+        QC_HIP(hipGraphLaunch(exec, s));
+        QC_HIP(hipEventRecord(e1, s));
+        QC_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        QC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        RunState rs{};
+        std::vector<DevStats> ist(count);
+        QC_HIP(hipMemcpy(rs.ctr, d_ctr.p, sizeof(rs.ctr), hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(&rs.clq, d_clq_ctr.p, sizeof(rs.clq), hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(&rs.err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(ist.data(), d_istats.p, (size_t)count * sizeof(DevStats), hipMemcpyDeviceToHost));
+        if (rs.err) {
+            set_error("persistent expansion failed (error flags %d: 1 capacity, 2 task too large, 4 queue hand-off)",
+                      rs.err);
+            status = (rs.err & 4) ? QC_EDEVICE : QC_EOVERFLOW;
+            break;
+        }
+        if (rep == 0) {
+            first = rs;
+            first_stats = ist;
+            first_ms = ms;
+        } else if (memcmp(first.ctr, rs.ctr, sizeof(rs.ctr)) != 0 || first.clq.items != rs.clq.items ||
+                   first.clq.slots != rs.clq.slots ||
+                   memcmp(first_stats.data(), ist.data(), (size_t)count * sizeof(DevStats)) != 0) {
+            set_error("graph replay %d differs from the first launch", rep);
+            status = QC_EDEVICE;
+        }
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    out.kernel_ms = ms;
-    out.levels = 1;
-    int err = 0;
-    QC_HIP(hipMemcpy(&err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) {
-        set_error("persistent expansion failed (error flags %d: 1 capacity, 2 task too large, 4 queue hand-off)", err);
-        return (err & 4) ? QC_EDEVICE : QC_EOVERFLOW;
-    }
-    StreamCtr cc;
-    QC_HIP(hipMemcpy(&cc, d_clq_ctr.p, sizeof(cc), hipMemcpyDeviceToHost));
+    if (status != QC_OK) return status;
+
+    // per-instance outputs (the last replay's buffers; replays are identical
+    // in every counter, the raw sets only in their order)
+    StreamCtr cc = first.clq;
     std::vector<CliqueRef> hr(cc.items);
     std::vector<uint32_t> hv(cc.slots);
     if (cc.items) QC_HIP(hipMemcpy(hr.data(), d_clq_refs.p, cc.items * sizeof(CliqueRef), hipMemcpyDeviceToHost));
     if (cc.slots) QC_HIP(hipMemcpy(hv.data(), d_clq_verts.p, cc.slots * 4, hipMemcpyDeviceToHost));
-    out.raw_off.reserve(cc.items + 1);
-    out.raw_v.reserve(cc.slots);
     for (auto& r : hr) {
-        out.raw_v.insert(out.raw_v.end(), hv.begin() + r.off, hv.begin() + r.off + r.len);
-        out.raw_off.push_back(out.raw_v.size());
+        EnumOut& o = outs[r.inst];
+        o.raw_v.insert(o.raw_v.end(), hv.begin() + r.off, hv.begin() + r.off + r.len);
+        o.raw_off.push_back(o.raw_v.size());
     }
-    QC_HIP(hipMemcpy(&out.ds, d_stats.p, sizeof(DevStats), hipMemcpyDeviceToHost));
+    for (int i = 0; i < count; i++) {
+        outs[i].ds = first_stats[i];
+        outs[i].kernel_ms = first_ms;
+        outs[i].levels = 1;
+    }
     return QC_OK;
 }
 

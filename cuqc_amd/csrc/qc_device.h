@@ -14,7 +14,8 @@
 struct TaskRef {
     uint64_t off;
     uint32_t t;
-    uint32_t depth;
+    uint16_t depth;
+    uint16_t inst;   // mining instance (batched launches)
     uint64_t bound;
     uint32_t b0;
     uint32_t b1;
@@ -34,7 +35,7 @@ struct __attribute__((aligned(16))) Vrec {
 struct CliqueRef {
     uint64_t off;
     uint32_t len;
-    uint32_t pad;
+    uint32_t inst;
 };
 
 // Counters of one output stream (children or cliques).
@@ -66,11 +67,17 @@ struct DevGraph {
     const uint32_t* orig; // root-local id -> input vertex id
     const int32_t* d2r;   // |N2(a) n root| per root vertex
     const int32_t* d1r;   // |N(a) n root| per root vertex
+    // search window (see qc_params)
+    long long root_b0, root_b1;
+    int32_t max_depth;
+    int32_t pad1;
 };
 
 // Arguments of one expansion launch.
 struct ExpandArgs {
-    DevGraph g;
+    DevGraph g;                      // single-instance launches
+    const DevGraph* graphs;          // batched launches: per-instance tables (NULL: use g)
+    DevStats* inst_stats;            // batched launches: per-instance counters
     const TaskRef* in_refs;
     const Vrec* in_verts;
     uint32_t n_in;
@@ -96,10 +103,7 @@ struct ExpandArgs {
     char* slab;
     unsigned long long slab_bytes;   // per wave
     uint32_t slab_tcap;              // vertices per slab
-
-    // search window (see qc_params)
-    long long root_b0, root_b1;
-    int32_t max_depth;
+    uint32_t slab_mwords;            // mask words per slab (root-space row words)
 
     // persistent mode: one FIFO queue of task refs over a bump-allocated
     // vertex pool; q_ready[i] publishes slot i; `pending` counts tasks pushed

@@ -66,6 +66,9 @@ typedef struct qc_params {
     const void* cost_model;  /* qc_model*, required when order == QC_ORDER_COST */
     int64_t root_begin;      /* search window: root branches [root_begin, root_end) in */
     int64_t root_end;        /*   the reference's branch order (main.cu:1094); end <= 0: all */
+    int32_t replays;         /* persistent mode: launch the captured hipGraph this many times and
+                                require identical counters from every replay; <= 1: once */
+    int32_t reserved1;
 } qc_params;
 
 typedef struct qc_stats {

@@ -101,7 +101,7 @@ def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dic
     tasks = [(wl, inst, a, b) for a, b in zip(cuts, cuts[1:])]
     tot = {k: 0 for k in STATS}
     raw = []
-    with ProcessPoolExecutor(jobs) as ex:
+    with ProcessPoolExecutor(int(os.environ.get("GOLDEN_JOBS", jobs))) as ex:
         for st, r in ex.map(_window_part, tasks):
             for k in STATS:
                 tot[k] += st[k] if k != "tasks" else st[k] - 1

@@ -114,3 +114,26 @@ def test_config0_window_split(cfg0, window):
     ref = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=window[0], root_end=window[1], max_depth=window[2])
     res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=4, window=window)
     _compare(res, ref, f"config0 window {window} split")
+
+
+def test_batched_small_instances():
+    """Many instances through one persistent launch: each instance's result
+    equals its own oracle run."""
+    cases = [(n, *b(), gm, ms) for n, b, gm, ms in SMALL_CASES]
+    g0 = qc.Graph.generate(0, 1)
+    cases.append(("config0", *g0.csr(), g0.gamma, g0.min_size))
+    _say(f"[gpu] batched {len(cases)} instances")
+    graphs = [qc.Graph.from_csr(off, adj) for _, off, adj, _, _ in cases]
+    res = qc.mine_batch(graphs, [c[3] for c in cases], [c[4] for c in cases], lds_tile=64, split_chunk=4)
+    for (name, off, adj, gamma, ms), r in zip(cases, res):
+        _compare(r, oracle.mine(off, adj, gamma, ms), f"batched {name}")
+
+
+def test_batched_graph_replays():
+    g0 = qc.Graph.generate(0, 1)
+    off, adj = g0.csr()
+    ref = oracle.mine(off, adj, g0.gamma, g0.min_size)
+    _say("[gpu] batched graph replays x4")
+    res = qc.mine_batch([g0, g0], [g0.gamma] * 2, [g0.min_size] * 2, lds_tile=128, replays=4)
+    for r in res:
+        _compare(r, ref, "replays")
