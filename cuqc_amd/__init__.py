@@ -20,7 +20,7 @@ from typing import Sequence
 import numpy as np
 
 __all__ = [
-    "QCError", "Graph", "Result", "mine", "mine_batch", "remove_nonmax", "device_available",
+    "QCError", "Graph", "Result", "CostModel", "mine", "mine_batch", "remove_nonmax", "device_available",
     "library_path", "load_library",
 ]
 
@@ -46,7 +46,7 @@ class _Params(C.Structure):
         ("order", C.c_int32), ("workgroups", C.c_int32), ("lds_tile", C.c_int32), ("split_chunk", C.c_int32),
         ("max_depth", C.c_int32), ("task_capacity", C.c_uint64), ("clique_capacity", C.c_uint64),
         ("cost_model", C.c_void_p), ("root_begin", C.c_int64), ("root_end", C.c_int64),
-        ("replays", C.c_int32), ("reserved1", C.c_int32),
+        ("replays", C.c_int32), ("trace", C.c_int32),
     ]
 
 
@@ -93,11 +93,21 @@ def load_library() -> C.CDLL:
         "qc_result_view": (C.c_int, [P, u64p, C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
         "qc_result_raw": (C.c_int, [P, u64p, C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
         "qc_result_stats": (C.c_int, [P, C.POINTER(_Stats)]),
+        "qc_result_trace": (C.c_int, [P, u64p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_int64)),
+                                      C.POINTER(C.POINTER(C.c_int64))]),
         "qc_result_write": (C.c_int, [P, C.c_char_p]),
         "qc_result_write_raw": (C.c_int, [P, C.c_char_p]),
         "qc_result_free": (None, [P]),
         "qc_remove_nonmax": (C.c_int, [C.c_uint64, P, P, PP]),
         "qc_degree_tables": (C.c_int, [C.c_double, C.c_int32, P, C.c_int32, P]),
+        "qc_model_train": (C.c_int, [P, P, C.c_uint64, C.c_int32, C.c_float, C.c_uint64, PP]),
+        "qc_model_predict": (C.c_int, [P, P, C.c_uint64, P]),
+        "qc_model_grad": (C.c_int, [P, P, P, C.c_uint64, P, P]),
+        "qc_model_create": (C.c_int, [C.c_uint64, PP]),
+        "qc_model_params": (C.c_int, [P, P, P, P, P]),
+        "qc_model_save": (C.c_int, [P, C.c_char_p]),
+        "qc_model_load": (C.c_int, [C.c_char_p, PP]),
+        "qc_model_free": (None, [P]),
         "qc_last_error": (C.c_char_p, []),
         "qc_version": (C.c_char_p, []),
         "qc_device_available": (C.c_int, []),
@@ -227,6 +237,9 @@ class Result:
     raw: list[np.ndarray]
     stats: dict
     status: int
+    trace_feats: np.ndarray | None = None     # (rows, 12) float32
+    trace_parent: np.ndarray | None = None    # row of the parent task, -1 root
+    trace_subtree: np.ndarray | None = None   # tasks in the subtree
 
     def write(self, path) -> None:
         write_sets(self.maximal, path)
@@ -255,13 +268,22 @@ def _result_from(h: C.c_void_p, rc: int) -> Result:
         stats = {n: int(getattr(st, n)) for n in _STAT_U64}
         stats.update(t_init_ms=st.t_init_ms, t_enum_ms=st.t_enum_ms, t_nonmax_ms=st.t_nonmax_ms,
                      t_kernel_ms=st.t_kernel_ms)
-        return Result(maximal, raw, stats, rc)
+        rows = C.c_uint64()
+        fp, pp, sp = C.POINTER(C.c_float)(), C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)()
+        _check(L.qc_result_trace(h, C.byref(rows), C.byref(fp), C.byref(pp), C.byref(sp)))
+        tf = tpar = tsub = None
+        if rows.value:
+            n = rows.value
+            tf = np.ctypeslib.as_array(fp, shape=(n, 12)).copy()
+            tpar = np.ctypeslib.as_array(pp, shape=(n,)).copy()
+            tsub = np.ctypeslib.as_array(sp, shape=(n,)).copy()
+        return Result(maximal, raw, stats, rc, tf, tpar, tsub)
     finally:
         L.qc_result_free(h)
 
 
 def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-            cost_model=None, split_chunk=0, window=None, replays=1) -> _Params:
+            cost_model=None, split_chunk=0, window=None, replays=1, trace=False) -> _Params:
     p = _Params()
     p.gamma = float(gamma)
     p.min_size = int(min_size)
@@ -277,12 +299,14 @@ def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task
     rb, re_, md = window if window is not None else (0, -1, -1)
     p.root_begin, p.root_end, p.max_depth = max(0, int(rb)), max(0, int(re_)), max(0, int(md))
     p.replays = int(replays)
+    p.trace = int(bool(trace))
     return p
 
 
 def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode: str = "plain",
          order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
-         clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, window=None, replays: int = 1) -> Result:
+         clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, window=None, replays: int = 1,
+         trace: bool = False) -> Result:
     """Mine maximal gamma-quasi-cliques of size >= min_size (search() + RemoveNonMax()).
 
     ``window=(root_begin, root_end, max_depth)`` restricts the search to root
@@ -291,7 +315,7 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
     than raised.
     """
     p = _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-                cost_model, split_chunk, window, replays)
+                cost_model, split_chunk, window, replays, trace)
     h = C.c_void_p()
     rc = _check(load_library().qc_mine(graph._h, C.byref(p), C.byref(h)), ok=(QC_OK, QC_EEMPTY))
     return _result_from(h, rc)
@@ -300,18 +324,91 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
 def mine_batch(graphs: Sequence[Graph], gammas: Sequence[float], min_sizes: Sequence[int], *, scheduling: int = 0,
                order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
                clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, windows=None,
-               replays: int = 1) -> list[Result]:
+               replays: int = 1, trace: bool = False) -> list[Result]:
     """Many instances through one persistent launch."""
     n = len(graphs)
     ps = (_Params * n)()
     for i in range(n):
         ps[i] = _params(gammas[i], min_sizes[i], scheduling, "persistent", order, lds_tile, workgroups,
                         task_capacity, clique_capacity, cost_model, split_chunk,
-                        windows[i] if windows is not None else None, replays)
+                        windows[i] if windows is not None else None, replays, trace)
     hs = (C.c_void_p * n)(*[g._h for g in graphs])
     out = (C.c_void_p * n)()
     _check(load_library().qc_mine_batch(hs, ps, n, out))
     return [_result_from(C.c_void_p(out[i]), QC_OK) for i in range(n)]
+
+
+NPARAMS = 5057
+NNORM = 29
+
+
+class CostModel:
+    """Task-cost MLP (12 -> 64 -> 64 -> 1, bf16 weights, fp32 accumulation)
+    trained and evaluated by the library's GPU kernels."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def create(cls, seed: int = 0) -> "CostModel":
+        h = C.c_void_p()
+        _check(load_library().qc_model_create(seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def train(cls, feats, log_target, steps: int = 300, lr: float = 3e-3, seed: int = 0) -> "CostModel":
+        X = np.ascontiguousarray(feats, np.float32)
+        y = np.ascontiguousarray(log_target, np.float32)
+        h = C.c_void_p()
+        _check(load_library().qc_model_train(X.ctypes.data, y.ctypes.data, len(y), steps, lr, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path) -> "CostModel":
+        h = C.c_void_p()
+        _check(load_library().qc_model_load(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    def save(self, path) -> None:
+        _check(load_library().qc_model_save(self._h, os.fsencode(path)))
+
+    def predict(self, feats) -> np.ndarray:
+        X = np.ascontiguousarray(feats, np.float32)
+        out = np.zeros(len(X), np.float32)
+        _check(load_library().qc_model_predict(self._h, X.ctypes.data, len(X), out.ctypes.data))
+        return out
+
+    def grad(self, feats, log_target) -> tuple[np.ndarray, float]:
+        X = np.ascontiguousarray(feats, np.float32)
+        y = np.ascontiguousarray(log_target, np.float32)
+        g = np.zeros(NPARAMS, np.float32)
+        loss = C.c_float()
+        _check(load_library().qc_model_grad(self._h, X.ctypes.data, y.ctypes.data, len(y), g.ctypes.data,
+                                            C.byref(loss)))
+        return g, loss.value
+
+    def params(self) -> tuple[np.ndarray, np.ndarray]:
+        p = np.zeros(NPARAMS, np.float32)
+        nrm = np.zeros(NNORM, np.float32)
+        _check(load_library().qc_model_params(self._h, p.ctypes.data, None, nrm.ctypes.data, None))
+        return p, nrm
+
+    def set_params(self, p=None, norm=None) -> None:
+        pp = np.ascontiguousarray(p, np.float32) if p is not None else None
+        nn = np.ascontiguousarray(norm, np.float32) if norm is not None else None
+        _check(load_library().qc_model_params(self._h, None, pp.ctypes.data if pp is not None else None, None,
+                                              nn.ctypes.data if nn is not None else None))
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                load_library().qc_model_free(self._h)
+        except Exception:
+            pass
 
 
 def degree_tables(gamma: float, n: int) -> tuple[np.ndarray, np.ndarray]:

@@ -24,7 +24,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["expand.hip", "root.hip", "driver.hip", "nonmax.hip", "persist.hip", "capi.hip"]
+HIP_SOURCES = ["expand.hip", "root.hip", "driver.hip", "nonmax.hip", "persist.hip", "mlp.hip", "capi.hip"]
 CXX_SOURCES = ["graph_io.cpp", "gen.cpp"]
 HEADERS = ["qc_host.h", "qc_device.h", "qc_internal.h"]
 

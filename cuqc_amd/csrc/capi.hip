@@ -83,6 +83,9 @@ int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0,
     int rc;
     r->raw_off = eo.raw_off;
     r->raw_v = eo.raw_v;
+    r->trace_feat = eo.trace_feat;
+    r->trace_parent = eo.trace_parent;
+    r->trace_subtree = eo.trace_subtree;
     double t_enum_end = now_ms();
     r->st.t_init_ms = t_init - t0;
     r->st.t_enum_ms = t_enum_end - t0;
@@ -312,6 +315,16 @@ int qc_result_raw(const qc_result* r, uint64_t* count, const uint64_t** offsets,
     if (count) *count = r->raw_off.size() - 1;
     if (offsets) *offsets = r->raw_off.data();
     if (verts) *verts = r->raw_v.data();
+    return QC_OK;
+}
+
+int qc_result_trace(const qc_result* r, uint64_t* rows, const float** feats, const int64_t** parent,
+                    const int64_t** subtree) {
+    if (!r) return set_error("NULL result"), QC_EINVAL;
+    if (rows) *rows = r->trace_parent.size();
+    if (feats) *feats = r->trace_feat.data();
+    if (parent) *parent = r->trace_parent.data();
+    if (subtree) *subtree = r->trace_subtree.data();
     return QC_OK;
 }
 

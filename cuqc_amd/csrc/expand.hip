@@ -48,6 +48,11 @@ __device__ __forceinline__ int wave_sum(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
     return v;
 }
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
 __device__ __forceinline__ long long wave_sum64(long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
@@ -169,6 +174,7 @@ struct Ctx {
     uint32_t depth;
     uint32_t cap;         // largest task the workspace holds
     uint32_t inst;        // instance of the task being expanded
+    long long slot;       // queue slot of the task being expanded (trace)
 };
 
 __device__ __forceinline__ int md(const Ctx& c, int i) {
@@ -654,7 +660,78 @@ __device__ __forceinline__ void q_key(int lab, int in, int ex, int l2, int id, u
     lo = ((uint64_t)(0x7FFFFFFFu - (uint32_t)l2) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)id);
 }
 
-__device__ void write_child(Ctx& c, int tc, int mem) {
+// Cost-model features of a record held in child positions [0, tc) (members
+// first): members, candidates, min/max/mean indeg, min/max/mean exdeg, depth,
+// ub - lb, mean |N2 n root| over candidates, size.  Means are
+// float(sum) / float(count), as in the oracle.
+__device__ void task_features(Ctx& c, int tc, int mem, int depth, int gap, float* f /* lane 0 */) {
+    WS& w = c.w;
+    int mi = INT32_MAX, ma = INT32_MIN, me = INT32_MAX, mx = INT32_MIN;
+    long long si = 0, se = 0, s2 = 0;
+    for (int i = c.lane; i < tc; i += WAVE) {
+        int in = w.cin[i], ex = w.cex[i];
+        mi = min(mi, in);
+        ma = max(ma, in);
+        me = min(me, ex);
+        mx = max(mx, ex);
+        si += in;
+        se += ex;
+        if (i >= mem) s2 += c.g->d2r[w.cid[i]];
+    }
+    mi = wave_min(mi);
+    ma = wave_max(ma);
+    me = wave_min(me);
+    mx = wave_max(mx);
+    si = wave_sum64(si);
+    se = wave_sum64(se);
+    s2 = wave_sum64(s2);
+    if (c.lane == 0) {
+        const int cnd = tc - mem;
+        f[0] = (float)mem;
+        f[1] = (float)cnd;
+        f[2] = (float)mi;
+        f[3] = (float)ma;
+        f[4] = (float)si / (float)tc;
+        f[5] = (float)me;
+        f[6] = (float)mx;
+        f[7] = (float)se / (float)tc;
+        f[8] = (float)depth;
+        f[9] = (float)gap;
+        f[10] = cnd > 0 ? (float)s2 / (float)cnd : 0.0f;
+        f[11] = (float)tc;
+    }
+}
+
+__device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Bucket of a child from the cost model (12 -> 64 -> 64 -> 1 ReLU MLP,
+// bf16 weights, fp32 accumulation): lane j evaluates hidden unit j of both
+// layers; the prediction is compared with the bucket thresholds.
+__device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
+    const ExpandArgs& a = *c.a;
+    const uint16_t* W = a.cm_w;
+    const float* nrm = a.cm_norm;
+    const int j = c.lane;
+    float h1 = bf16f(W[12 * 64 + j]);
+    for (int k = 0; k < 12; k++) {
+        float f = __shfl(feat[k], 0, WAVE);
+        float z = (log1pf(fmaxf(f, 0.0f)) - nrm[k]) / nrm[12 + k];
+        h1 = fmaf(z, bf16f(W[k * 64 + j]), h1);
+    }
+    h1 = fmaxf(h1, 0.0f);
+    const int ow2 = 12 * 64 + 64;
+    float h2 = bf16f(W[ow2 + 64 * 64 + j]);
+    for (int i = 0; i < 64; i++) h2 = fmaf(__shfl(h1, i, WAVE), bf16f(W[ow2 + i * 64 + j]), h2);
+    h2 = fmaxf(h2, 0.0f);
+    const int ow3 = ow2 + 64 * 64 + 64;
+    float y = wave_sum_f(h2 * bf16f(W[ow3 + j])) + bf16f(W[ow3 + 64]);
+    int b = 0;
+    for (int q = 0; q < a.n_buckets - 1 && q < 3; q++)
+        if (y > nrm[26 + q]) b = q + 1;
+    return uni(b);
+}
+
+__device__ void write_child(Ctx& c, int tc, int mem, int gap) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
     if (c.g->max_depth >= 0 && (int)c.depth + 1 > c.g->max_depth) {
@@ -669,10 +746,14 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
     sort_keys(c, w.khi, w.klo, tc);
     const bool q = a.queue_mode != 0;
     int s = (uint32_t)tc <= a.small_max ? 0 : 1;
+    int bucket = 0;
+    float feat[12];
+    if (q && (a.trace_feat || a.n_buckets > 1)) task_features(c, tc, mem, (int)c.depth + 1, gap, feat);
+    if (q && a.n_buckets > 1) bucket = cost_bucket(c, feat);
     unsigned long long ri = 0, si = 0;
     if (c.lane == 0) {
         if (q) {
-            ri = atomicAdd(a.q_tail, 1ull);
+            ri = atomicAdd(&a.q_tail[bucket], 1ull);
             si = atomicAdd(a.pool_top, (unsigned long long)tc);
         } else {
             ri = atomicAdd(&a.out_ctr[s].items, 1ull);
@@ -685,10 +766,13 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
         if (c.lane == 0) {
             atomicOr(a.error, 1);
             // the slot stays unpublished: mark it done so no consumer waits
-            if (q && ri < a.q_cap) __hip_atomic_store(&a.q_ready[ri], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (q && ri < a.q_cap)
+                __hip_atomic_store(&a.q_ready[bucket * a.q_cap + ri], 2u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         return;
     }
+    if (q) ri += (unsigned long long)bucket * a.q_cap;  // global slot
     Vrec* dst = (q ? a.pool : a.out_verts[s]) + si;
     for (int i = c.lane; i < tc; i += WAVE) {
         uint64_t hi = w.khi[i], lo = w.klo[i];
@@ -717,6 +801,7 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
         r.b0 = 0;
         r.b1 = 0;
         if (q) {
+            if (a.trace_parent) a.trace_parent[ri] = c.slot;
             // count the task before it becomes visible, then publish
             atomicAdd((unsigned long long*)a.pending, 1ull);
             a.q_refs[ri] = r;
@@ -724,6 +809,8 @@ __device__ void write_child(Ctx& c, int tc, int mem) {
             a.out_refs[s][ri] = r;
         }
     }
+    if (q && a.trace_feat && c.lane == 0)
+        for (int k = 0; k < 12; k++) a.trace_feat[ri * 12 + k] = feat[k];
     if (q) {
         __threadfence();
         if (c.lane == 0) __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -957,16 +1044,19 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         __threadfence();
         for (int b0 = bbeg; b0 < hi; b0 += ch) {
             if (c.lane == 0) {
-                unsigned long long ri = atomicAdd(a.q_tail, 1ull);
+                const int top = a.n_buckets - 1;
+                unsigned long long ri = atomicAdd(&a.q_tail[top], 1ull);
                 if (ri >= a.q_cap) {
                     atomicOr(a.error, 1);
                 } else {
+                    ri += (unsigned long long)top * a.q_cap;
                     TaskRef r;
                     r.off = slot;
                     r.t = (uint32_t)t0;
                     r.depth = ref.depth;
                     r.inst = ref.inst;
-                    r.bound = 0;
+                    r.bound = (uint64_t)c.slot;  // the task the item belongs to
+                    if (a.trace_parent) a.trace_parent[ri] = -2;
                     r.b0 = (uint32_t)b0;
                     r.b1 = (uint32_t)min(b0 + ch, hi);
                     atomicAdd((unsigned long long*)a.pending, 1ull);
@@ -1034,7 +1124,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         if (r == 2) continue;
         if (m >= c.g->min_size) check_for_clique(c, m);
         if (r == 1) continue;
-        if (cc > 0) write_child(c, tc, m);
+        if (cc > 0) write_child(c, tc, m, ub - lb);
         wsync();
     }
 #undef PM
@@ -1099,6 +1189,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
+    c.slot = -1;
     c.lane = lane_id();
     c.w = make_ws(lds, TL, true, (TL + 63) / 64);
     c.cap = TL;
@@ -1112,6 +1203,7 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
+    c.slot = -1;
     c.lane = lane_id();
     char* base = a.slab + (size_t)blockIdx.x * a.slab_bytes;
     c.w = make_ws(base, a.slab_tcap, false, a.slab_mwords);
@@ -1134,6 +1226,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
+    c.slot = -1;
     c.lane = lane_id();
     const WS wl = make_ws(lds, TL, true, (TL + 63) / 64);
     const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, a.slab_mwords);
@@ -1150,9 +1243,15 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                 // wave leaves (the host reports the error)
                 state = 2;
             } else {
-                h = __hip_atomic_load(a.q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                unsigned long long t = __hip_atomic_load(a.q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (h < t && h < a.q_cap && atomicCAS(a.q_head, h, h + 1) == h) state = 1;
+                // highest non-empty bucket first
+                for (int b = a.n_buckets - 1; b >= 0 && state == 0; b--) {
+                    unsigned long long hb = __hip_atomic_load(&a.q_head[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    unsigned long long tb = __hip_atomic_load(&a.q_tail[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (hb < tb && hb < a.q_cap && atomicCAS(&a.q_head[b], hb, hb + 1) == hb) {
+                        state = 1;
+                        h = hb + (unsigned long long)b * a.q_cap;
+                    }
+                }
             }
         }
         state = uni(__shfl(state, 0, WAVE));
@@ -1179,6 +1278,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                 c.g = &a.graphs[ref.inst];
                 c.inst = ref.inst;
             }
+            c.slot = ref.b1 > 0 ? (long long)ref.bound : (long long)h;
             if (ref.t <= (uint32_t)TL) {
                 c.w = wl;
                 c.cap = TL;

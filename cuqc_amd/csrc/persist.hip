@@ -21,21 +21,31 @@ __global__ void expand_persistent(const ExpandArgs*);
 
 // Queue state for a fresh run: roots in slots [0, nroots), their records at
 // the start of the pool, every other ready flag cleared, counters reset.
+// Roots go to the top bucket nb - 1; ctr = heads[nb], tails[nb], pool top,
+// pending.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
-                              uint64_t root_slots, TaskRef* q_refs, unsigned int* q_ready, uint64_t q_cap, Vrec* pool,
-                              unsigned long long* ctr, StreamCtr* clq_ctr, DevStats* stats, DevStats* inst_stats,
-                              uint32_t n_inst, int* error) {
+                              uint64_t root_slots, TaskRef* q_refs, unsigned int* q_ready, uint64_t q_cap, int nb,
+                              Vrec* pool, unsigned long long* ctr, StreamCtr* clq_ctr, DevStats* stats,
+                              DevStats* inst_stats, uint32_t n_inst, int* error, const float* __restrict__ root_feat,
+                              float* trace_feat, long long* trace_parent) {
     const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = tid; i < q_cap; i += nt) q_ready[i] = i < nroots ? 1u : 0u;
-    for (uint64_t i = tid; i < nroots; i += nt) q_refs[i] = roots[i];
+    const uint64_t top = (uint64_t)(nb - 1) * q_cap;
+    for (uint64_t i = tid; i < (uint64_t)nb * q_cap; i += nt) q_ready[i] = (i >= top && i < top + nroots) ? 1u : 0u;
+    for (uint64_t i = tid; i < nroots; i += nt) q_refs[top + i] = roots[i];
+    if (trace_feat) {
+        for (uint64_t i = tid; i < (uint64_t)nroots * 12; i += nt) trace_feat[i] = root_feat[i];
+        for (uint64_t i = tid; i < nroots; i += nt) trace_parent[i] = -1;
+    }
     for (uint64_t i = tid; i < root_slots; i += nt) pool[i] = root_verts[i];
     for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (tid == 0) {
-        ctr[0] = 0;            // head
-        ctr[1] = nroots;       // tail
-        ctr[2] = root_slots;   // pool top
-        ctr[3] = nroots;       // pending
+        for (int b = 0; b < nb; b++) {
+            ctr[b] = 0;                                  // head
+            ctr[nb + b] = b == nb - 1 ? nroots : 0;      // tail
+        }
+        ctr[2 * nb] = root_slots;      // pool top
+        ctr[2 * nb + 1] = nroots;      // pending
         clq_ctr->items = 0;
         clq_ctr->slots = 0;
         *stats = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -57,7 +67,7 @@ size_t slab_bytes(uint32_t tcap, uint32_t rw) {
 }
 
 struct RunState {
-    unsigned long long ctr[4];
+    unsigned long long ctr[10];
     StreamCtr clq;
     int err;
 };
@@ -74,6 +84,12 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     }
     const qc_params& p = params[0];
     const int TL = p.lds_tile == 64 ? 64 : p.lds_tile == 256 ? 256 : 128;
+    const bool cost = p.order == QC_ORDER_COST;
+    const int nb = cost ? 4 : 1;
+    if (cost && p.trace) {
+        set_error("task traces are recorded with FIFO order only");
+        return QC_EINVAL;
+    }
     const void* kern = TL == 64    ? (const void*)qcdev::expand_persistent<64>
                        : TL == 128 ? (const void*)qcdev::expand_persistent<128>
                                    : (const void*)qcdev::expand_persistent<256>;
@@ -82,6 +98,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     std::vector<TaskRef> roots;
     std::vector<Vrec> root_verts;
     std::vector<DevGraph> graphs(count);
+    std::vector<float> root_feat;
     int maxR = 1, maxrw = 1;
     for (int i = 0; i < count; i++) {
         const Instance& I = *insts[i];
@@ -95,6 +112,19 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         TaskRef r{root_verts.size(), (uint32_t)I.R, 0, (uint16_t)i, I.root_bound, 0, 0};
         roots.push_back(r);
         root_verts.insert(root_verts.end(), I.root.begin(), I.root.end());
+        // root features (no members: indeg 0, every vertex a candidate)
+        int me = INT32_MAX, mx = INT32_MIN;
+        long long se = 0, s2 = 0;
+        for (const Vrec& v : I.root) {
+            me = std::min(me, v.exdeg);
+            mx = std::max(mx, v.exdeg);
+            se += v.exdeg;
+            s2 += I.d2r[v.id];
+        }
+        const float R = (float)I.R;
+        const float f[12] = {0.0f, R, 0.0f, 0.0f, 0.0f / R, (float)me, (float)mx, (float)se / R, 0.0f, R,
+                             (float)s2 / R, R};
+        root_feat.insert(root_feat.end(), f, f + 12);
     }
     if (roots.empty()) return QC_OK;
 
@@ -116,15 +146,28 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     const uint64_t pool_cap = std::max<uint64_t>(
         root_verts.size() + 1024,
         p.task_capacity ? p.task_capacity : std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec));
-    const uint64_t q_cap = std::max<uint64_t>(roots.size() + 1024, std::min<uint64_t>(pool_cap / 8 + 1024, 1ull << 26));
+    // per bucket
+    const uint64_t q_cap =
+        std::max<uint64_t>(roots.size() + 1024, std::min<uint64_t>(pool_cap / 8 + 1024, (1ull << 26) / nb));
     const uint64_t clq_slots =
         p.clique_capacity ? p.clique_capacity : std::min<uint64_t>(budget / 16 / 4, (uint64_t)1 << 30);
     const uint64_t clq_refs = std::min<uint64_t>(clq_slots / 4 + 1024, (uint64_t)1 << 26);
 
     DBuf d_pool, d_q, d_ready, d_ctr, d_slab, d_clq_refs, d_clq_verts, d_clq_ctr, d_stats, d_istats, d_err, d_args,
-        d_graphs, d_roots, d_root_verts;
-    if ((rc = d_pool.alloc(pool_cap * sizeof(Vrec))) || (rc = d_q.alloc(q_cap * sizeof(TaskRef))) ||
-        (rc = d_ready.alloc(q_cap * 4)) || (rc = d_ctr.alloc(4 * 8)) || (rc = d_slab.alloc((size_t)grid * slab_b)) ||
+        d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_rfeat;
+    const bool trace = p.trace != 0;
+    if (trace) {
+        if ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8)) ||
+            (rc = d_rfeat.alloc(root_feat.size() * sizeof(float))))
+            return rc;
+        QC_HIP(hipMemcpyAsync(d_rfeat.p, root_feat.data(), root_feat.size() * sizeof(float), hipMemcpyHostToDevice,
+                              s));
+    }
+    DBuf d_cmw, d_cmn;
+    if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
+    if ((rc = d_pool.alloc(pool_cap * sizeof(Vrec))) || (rc = d_q.alloc(nb * q_cap * sizeof(TaskRef))) ||
+        (rc = d_ready.alloc(nb * q_cap * 4)) || (rc = d_ctr.alloc(10 * 8)) ||
+        (rc = d_slab.alloc((size_t)grid * slab_b)) ||
         (rc = d_clq_refs.alloc(clq_refs * sizeof(CliqueRef))) || (rc = d_clq_verts.alloc(clq_slots * 4)) ||
         (rc = d_clq_ctr.alloc(sizeof(StreamCtr))) || (rc = d_stats.alloc(sizeof(DevStats))) ||
         (rc = d_istats.alloc((size_t)count * sizeof(DevStats))) || (rc = d_err.alloc(sizeof(int))) ||
@@ -161,12 +204,17 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     A.q_refs = d_q.as<TaskRef>();
     A.q_ready = d_ready.as<unsigned int>();
     A.q_head = d_ctr.as<unsigned long long>() + 0;
-    A.q_tail = d_ctr.as<unsigned long long>() + 1;
-    A.pool_top = d_ctr.as<unsigned long long>() + 2;
-    A.pending = (long long*)(d_ctr.as<unsigned long long>() + 3);
+    A.q_tail = d_ctr.as<unsigned long long>() + nb;
+    A.pool_top = d_ctr.as<unsigned long long>() + 2 * nb;
+    A.pending = (long long*)(d_ctr.as<unsigned long long>() + 2 * nb + 1);
+    A.n_buckets = nb;
+    A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
+    A.cm_norm = cost ? d_cmn.as<float>() : nullptr;
     A.q_cap = q_cap;
     A.pool = d_pool.as<Vrec>();
     A.pool_cap = pool_cap;
+    A.trace_feat = trace ? d_tfeat.as<float>() : nullptr;
+    A.trace_parent = trace ? (long long*)d_tpar.p : nullptr;
     QC_HIP(hipMemcpyAsync(d_args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
     QC_HIP(hipStreamSynchronize(s));
 
@@ -179,9 +227,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
                        d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), d_q.as<TaskRef>(),
-                       d_ready.as<unsigned int>(), q_cap, d_pool.as<Vrec>(), d_ctr.as<unsigned long long>(),
+                       d_ready.as<unsigned int>(), q_cap, nb, d_pool.as<Vrec>(), d_ctr.as<unsigned long long>(),
                        d_clq_ctr.as<StreamCtr>(), d_stats.as<DevStats>(), d_istats.as<DevStats>(),
-                       (uint32_t)count, d_err.as<int>());
+                       (uint32_t)count, d_err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent);
     hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, 0, s);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -255,6 +303,29 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         outs[i].ds = first_stats[i];
         outs[i].kernel_ms = first_ms;
         outs[i].levels = 1;
+    }
+    if (trace) {
+        // rows = expanded tasks (split items excluded), per instance; a
+        // child's slot is always above its parent's, so one backward pass
+        // accumulates subtree sizes
+        const uint64_t tail = std::min<uint64_t>(first.ctr[nb], q_cap);
+        std::vector<float> tf(tail * 12);
+        std::vector<long long> tp(tail);
+        std::vector<TaskRef> qr(tail);
+        QC_HIP(hipMemcpy(tf.data(), d_tfeat.p, tail * 12 * sizeof(float), hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(tp.data(), d_tpar.p, tail * 8, hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(qr.data(), d_q.p, tail * sizeof(TaskRef), hipMemcpyDeviceToHost));
+        std::vector<int64_t> sub(tail, 1), row(tail, -1);
+        for (uint64_t i = tail; i-- > 0;)
+            if (tp[i] >= 0) sub[tp[i]] += sub[i];
+        for (uint64_t i = 0; i < tail; i++) {
+            if (tp[i] == -2) continue;
+            EnumOut& o = outs[qr[i].inst];
+            row[i] = (int64_t)o.trace_parent.size();
+            o.trace_feat.insert(o.trace_feat.end(), tf.begin() + i * 12, tf.begin() + i * 12 + 12);
+            o.trace_parent.push_back(tp[i] >= 0 ? row[tp[i]] : -1);
+            o.trace_subtree.push_back(sub[i]);
+        }
     }
     return QC_OK;
 }

@@ -105,9 +105,13 @@ struct ExpandArgs {
     uint32_t slab_tcap;              // vertices per slab
     uint32_t slab_mwords;            // mask words per slab (root-space row words)
 
-    // persistent mode: one FIFO queue of task refs over a bump-allocated
-    // vertex pool; q_ready[i] publishes slot i; `pending` counts tasks pushed
-    // and not yet finished (0 = search complete)
+    // persistent mode: n_buckets FIFO queues of task refs (bucket b owns
+    // slots [b q_cap, (b+1) q_cap) of q_refs / q_ready and the counters
+    // q_head[b], q_tail[b]) over a bump-allocated vertex pool; q_ready[i]
+    // publishes slot i; `pending` counts tasks pushed and not yet finished
+    // (0 = search complete).  With a cost model, a child goes to the bucket
+    // of its predicted log subtree size and waves pop the highest non-empty
+    // bucket first.
     int32_t queue_mode;
     int32_t split_chunk;             // branches per split item (0: no splitting)
     uint32_t split_min_t;            // split tasks above this size
@@ -120,6 +124,17 @@ struct ExpandArgs {
     unsigned long long* pool_top;
     unsigned long long pool_cap;
     long long* pending;
+
+    // task trace (persistent mode, NULL = off): features of the task in
+    // queue slot i at trace_feat[12 i], its parent's slot at trace_parent[i]
+    // (-1 root, -2 split item)
+    float* trace_feat;
+    long long* trace_parent;
+
+    int32_t n_buckets;               // 1 = FIFO
+    int32_t pad2;
+    const uint16_t* cm_w;            // cost model: bf16 weights (5057)
+    const float* cm_norm;            // mu[12] sd[12] ymu ysd thresholds[3]
 
     DevStats* stats;
     int* error;  // bit 0: output overflow, bit 1: task above slab capacity, bit 2: queue wait limit

@@ -18,6 +18,9 @@ struct qc_result {
     std::vector<uint64_t> raw_off{0};
     std::vector<uint32_t> raw_v;
     qc_stats st{};
+    // task trace (qc_params.trace)
+    std::vector<float> trace_feat;
+    std::vector<int64_t> trace_parent, trace_subtree;
 };
 
 namespace qc {
@@ -91,11 +94,16 @@ int build_instance(const qc_graph* g, double gamma, int min_size, Instance& inst
 struct EnumOut {
     std::vector<uint64_t> raw_off{0};
     std::vector<uint32_t> raw_v;
+    std::vector<float> trace_feat;
+    std::vector<int64_t> trace_parent, trace_subtree;
     DevStats ds{};
     double kernel_ms = 0;
     uint64_t levels = 0;
 };
 int enumerate_plain(const Instance& inst, const qc_params& p, EnumOut& out, hipStream_t s);
+
+// cost model weights / norms in device memory (mlp.hip)
+int model_device_params(const qc_model* m, DBuf& pb, DBuf& norm, hipStream_t s);
 
 // RemoveNonMax on the device (nonmax.hip)
 int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uint32_t>& v, std::vector<uint64_t>& mo,

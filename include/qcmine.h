@@ -68,7 +68,8 @@ typedef struct qc_params {
     int64_t root_end;        /*   the reference's branch order (main.cu:1094); end <= 0: all */
     int32_t replays;         /* persistent mode: launch the captured hipGraph this many times and
                                 require identical counters from every replay; <= 1: once */
-    int32_t reserved1;
+    int32_t trace;           /* persistent mode: record the cost-model features of every expanded
+                                task and its subtree size (qc_result_trace) */
 } qc_params;
 
 typedef struct qc_stats {
@@ -152,6 +153,14 @@ int qc_result_stats(const qc_result* r, qc_stats* s);
  * then "\n".  QC_EEMPTY (and no file) when the result holds no sets. */
 int qc_result_write(const qc_result* r, const char* path);
 
+/* Task trace of a run with qc_params.trace set (no reference counterpart):
+ * `rows` expanded tasks, 12 features each (members, candidates, min/max/mean
+ * indeg, min/max/mean exdeg, depth, ub - lb, mean |N2 n root| over
+ * candidates, size), the row of the parent task (-1 for the root) and the
+ * number of tasks in the subtree rooted at the task. */
+int qc_result_trace(const qc_result* r, uint64_t* rows, const float** feats, const int64_t** parent,
+                    const int64_t** subtree);
+
 /* temp.txt-compatible raw dump (flush_cliques, main.cu:1234-1246). */
 int qc_result_write_raw(const qc_result* r, const char* path);
 
@@ -162,6 +171,35 @@ void qc_result_free(qc_result* r);
 /* Replaces RemoveNonMax(temp, out) (main.cu:4939-4961) for caller-provided
  * raw sets: device canonicalisation, de-duplication and subset removal. */
 int qc_remove_nonmax(uint64_t count, const uint64_t* offsets, const uint32_t* verts, qc_result** out);
+
+/* ---- task-cost model (no reference counterpart) ------------------------
+ * A 12 -> 64 -> 64 -> 1 ReLU MLP with bfloat16 weights and fp32 accumulation
+ * that predicts log(subtree size) of a task from its qc_result_trace
+ * features; trained on the GPU.  Passed as qc_params.cost_model with
+ * order = QC_ORDER_COST it orders the persistent kernel's work; mined sets
+ * never depend on it. */
+typedef struct qc_model qc_model;
+#define QC_MODEL_NPARAMS 5057   /* W1[12][64] b1[64] W2[64][64] b2[64] W3[64] b3 */
+#define QC_MODEL_NNORM 29       /* mu[12] sd[12] ymu ysd thresholds[3] */
+
+/* Adam on the mean squared error of standardized log targets, full batch,
+ * `steps` updates; norms are fitted to the data. */
+int qc_model_train(const float* feats, const float* log_target, uint64_t n, int32_t steps, float lr, uint64_t seed,
+                   qc_model** out);
+/* Predicted log(subtree size) for n feature rows. */
+int qc_model_predict(const qc_model* m, const float* feats, uint64_t n, float* out);
+/* Gradient of the loss (mean squared error on standardized targets) with
+ * respect to the QC_MODEL_NPARAMS parameters, computed by the GPU kernels. */
+int qc_model_grad(const qc_model* m, const float* feats, const float* log_target, uint64_t n, float* grad,
+                  float* loss);
+/* Untrained model: seeded He initialisation, identity norms. */
+int qc_model_create(uint64_t seed, qc_model** out);
+/* Read / replace fp32 parameters and the QC_MODEL_NNORM normalisation values
+ * (any pointer may be NULL); the bf16 copy follows the fp32 values. */
+int qc_model_params(qc_model* m, float* get, const float* set, float* norm_get, const float* norm_set);
+int qc_model_save(const qc_model* m, const char* path);
+int qc_model_load(const char* path, qc_model** out);
+void qc_model_free(qc_model* m);
 
 /* ---- tables ---------------------------------------------------------- */
 

@@ -515,7 +515,7 @@ struct Oracle {
                 for (auto& x : child) x.l2 = 0;
                 st.children++;
                 st.sum_t_written += tt;
-                emit_child(std::move(child));
+                emit_child(std::move(child), ub - lb);
             }
         }
     }
@@ -664,31 +664,41 @@ struct Oracle {
         }
     }
 
-    void features(const std::vector<Vtx>& rec, int depth, float* f) {
+    // Task features for the cost model (12 values), computed from a record
+    // when it is written: members, candidates, min/max/mean indeg and exdeg
+    // over all vertices, depth, ub - lb of the bounds that admitted the
+    // record (candidates for the root), mean |N2 n root| over candidates,
+    // and the record size.  Means are float32(sum) / float32(count).
+    std::vector<int> d2r;
+    void features(const std::vector<Vtx>& rec, int depth, int gap, float* f) {
         int t = (int)rec.size();
         int nm = 0;
         while (nm < t && rec[nm].label == 1) nm++;
-        float mi = 1e30f, ma = -1e30f, si = 0, me = 1e30f, mx = -1e30f, se = 0;
-        for (auto& x : rec) {
-            mi = std::min(mi, (float)x.indeg);
-            ma = std::max(ma, (float)x.indeg);
+        int mi = INT32_MAX, ma = INT32_MIN, me = INT32_MAX, mx = INT32_MIN;
+        long long si = 0, se = 0, s2 = 0;
+        for (int k = 0; k < t; k++) {
+            const Vtx& x = rec[k];
+            mi = std::min(mi, x.indeg);
+            ma = std::max(ma, x.indeg);
             si += x.indeg;
-            me = std::min(me, (float)x.exdeg);
-            mx = std::max(mx, (float)x.exdeg);
+            me = std::min(me, x.exdeg);
+            mx = std::max(mx, x.exdeg);
             se += x.exdeg;
+            if (k >= nm) s2 += d2r[x.id];
         }
-        f[0] = nm;
-        f[1] = t - nm;
-        f[2] = mi;
-        f[3] = ma;
-        f[4] = si / t;
-        f[5] = me;
-        f[6] = mx;
-        f[7] = se / t;
-        f[8] = depth;
-        f[9] = 0;
-        f[10] = 0;
-        f[11] = t;
+        int c = t - nm;
+        f[0] = (float)nm;
+        f[1] = (float)c;
+        f[2] = (float)mi;
+        f[3] = (float)ma;
+        f[4] = (float)si / (float)t;
+        f[5] = (float)me;
+        f[6] = (float)mx;
+        f[7] = (float)se / (float)t;
+        f[8] = (float)depth;
+        f[9] = (float)gap;
+        f[10] = c > 0 ? (float)s2 / (float)c : 0.0f;
+        f[11] = (float)t;
     }
 
     int run() {
@@ -704,30 +714,45 @@ struct Oracle {
             std::vector<Vtx> rec;
             int depth;
             int64_t tid;
+            float f[12];
         };
+        if (trace) {
+            d2r.assign(n, 0);
+            for (int v = 0; v < n; v++) d2r[v] = (int)(r2off[v + 1] - r2off[v]);
+        }
         std::vector<Item> stack;
-        stack.push_back({std::move(root), 0, -1});
+        {
+            Item r0{std::move(root), 0, -1, {}};
+            if (trace) {
+                features(r0.rec, 0, (int)r0.rec.size(), r0.f);
+            }
+            stack.push_back(std::move(r0));
+        }
         std::vector<Item> kids;
         while (!stack.empty()) {
             Item it = std::move(stack.back());
             stack.pop_back();
-            int64_t my = -1;
-            if (trace) {
-                TraceRow r;
-                features(it.rec, it.depth, r.f);
-                r.parent = it.tid;
-                my = (int64_t)trace_rows.size();
-                trace_rows.push_back(r);
-            }
             if ((uint64_t)it.depth > st.max_depth) st.max_depth = it.depth;
             if (max_depth >= 0 && it.depth > max_depth) {
                 st.beyond++;
                 continue;
             }
+            int64_t my = -1;
+            if (trace) {
+                TraceRow r;
+                memcpy(r.f, it.f, sizeof(r.f));
+                r.parent = it.tid;
+                my = (int64_t)trace_rows.size();
+                trace_rows.push_back(r);
+            }
             cur_depth = it.depth;
             kids.clear();
             int d = it.depth + 1;
-            expand_task(it.rec, [&](std::vector<Vtx>&& c) { kids.push_back({std::move(c), d, my}); });
+            expand_task(it.rec, [&](std::vector<Vtx>&& c, int gap) {
+                Item k{std::move(c), d, my, {}};
+                if (trace) features(k.rec, d, gap, k.f);
+                kids.push_back(std::move(k));
+            });
             // children are processed in branch order (DFS preorder): the
             // first child written by the root is the next task processed,
             // as in the reference's second host level.

@@ -137,3 +137,24 @@ def test_batched_graph_replays():
     res = qc.mine_batch([g0, g0], [g0.gamma] * 2, [g0.min_size] * 2, lds_tile=128, replays=4)
     for r in res:
         _compare(r, ref, "replays")
+
+
+def _trace_rows(feats, subtree):
+    return sorted((tuple(float(x) for x in f), int(s)) for f, s in zip(feats, subtree))
+
+
+@pytest.mark.parametrize("chunk", [-1, 4])
+def test_trace_matches_oracle(cfg0, chunk):
+    """The GPU task trace (features + subtree sizes) equals the oracle's."""
+    g, _ = cfg0
+    off, adj = g.csr()
+    ref = oracle.mine(off, adj, g.gamma, g.min_size, trace=True)
+    sub = np.ones(len(ref["trace_parent"]), np.int64)
+    for i in range(len(sub) - 1, -1, -1):
+        p = ref["trace_parent"][i]
+        if p >= 0:
+            sub[p] += sub[i]
+    _say(f"[gpu] trace chunk={chunk}")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, trace=True)
+    assert len(res.trace_subtree) == ref["stats"]["tasks"]
+    assert _trace_rows(res.trace_feats, res.trace_subtree) == _trace_rows(ref["trace_feats"], sub)
