@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
 def set_digest(sets) -> str:
     h = hashlib.sha256()
     for s in sets:
@@ -78,15 +82,18 @@ def run_form(qc, g, w, form: str, steps: int, warmup: int):
         kw.update(mode="plain")
     else:
         kw.update(mode="persistent")
-    for _ in range(warmup):
-        qc.mine(g, g.gamma, g.min_size, 0, **kw)
+    for i in range(warmup):
+        r = qc.mine(g, g.gamma, g.min_size, 0, **kw)
+        _log(f"[bench] {form} warmup {i + 1}/{warmup}: enum {r.stats['t_enum_ms']:.1f} ms")
     enum_ms, wall_ms, kern_ms, res = [], [], [], None
-    for _ in range(steps):
+    for i in range(steps):
         t0 = time.perf_counter()
         res = qc.mine(g, g.gamma, g.min_size, 0, **kw)
         wall_ms.append((time.perf_counter() - t0) * 1e3)
         enum_ms.append(res.stats["t_enum_ms"])
         kern_ms.append(res.stats["t_kernel_ms"])
+        _log(f"[bench] {form} step {i + 1}/{steps}: enum {enum_ms[-1]:.1f} ms, kernel {kern_ms[-1]:.1f} ms, "
+             f"{res.stats['tasks']} tasks")
     return res, enum_ms, wall_ms, kern_ms
 
 
@@ -96,7 +103,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=1)
-    ap.add_argument("--forms", default="plain")
+    ap.add_argument("--forms", default="plain,persistent")
+    ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
@@ -114,7 +122,9 @@ def main() -> int:
     out = {}
     digest = None
     for f in forms:
-        res, enum_ms, wall_ms, kern_ms = run_form(qc, g, w, f, a.steps, a.warmup)
+        # the plain form is the comparison point: one untimed-warmup-free step
+        st_, wu_ = (a.plain_steps, 0) if f == "plain" else (a.steps, a.warmup)
+        res, enum_ms, wall_ms, kern_ms = run_form(qc, g, w, f, st_, wu_)
         d = set_digest(res.maximal)
         if digest is not None and d != digest:
             print(f"maximal sets differ between forms ({f})", file=sys.stderr)

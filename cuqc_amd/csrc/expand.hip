@@ -753,7 +753,7 @@ __device__ void write_child(Ctx& c, int tc, int mem, int gap) {
     unsigned long long ri = 0, si = 0;
     if (c.lane == 0) {
         if (q) {
-            ri = atomicAdd(&a.q_tail[bucket], 1ull);
+            ri = atomicAdd(&a.q_tail[bucket * QSTRIDE], 1ull);
             si = atomicAdd(a.pool_top, (unsigned long long)tc);
         } else {
             ri = atomicAdd(&a.out_ctr[s].items, 1ull);
@@ -1032,12 +1032,49 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             v.exdeg = w.pex[i];
             a.pool[slot + i] = v;
         }
+        // Removal k (branch b = k) takes position t0 - k.  Member i fails at
+        // the removal of its need_i-th neighbour counted from the end, with
+        // need_i = max(pin + pex - mindeg(nm) + 1, 1); kb = the first such
+        // removal over all members (bend: none before the window's end).
+        const int mdg = get_mindeg(c, nm);
         int kb = bend;
-        for (int b = 1; b < bend; b++) {
-            if (remove_step(true)) {
-                kb = b;
-                break;
+        for (int i = 0; i < nm; i++) {
+            const int need = max(w.pin[i] + w.pex[i] - mdg + 1, 1);
+            const int row = PM(i);
+            int cnt = 0, ki = INT32_MAX;
+            for (int k0 = 1; k0 < kb && ki == INT32_MAX; k0 += WAVE) {
+                const int k = k0 + c.lane;
+                const bool nb_ = k < kb && gbit(c.M1, c.W, row, PM(t0 - k));
+                const uint64_t bm = ballot(nb_);
+                const int pc = __popcll(bm);
+                if (cnt + pc >= need) {
+                    const uint64_t hit = ballot(nb_ && cnt + lanes_below(bm) + 1 == need);
+                    ki = k0 + __builtin_ctzll(hit);
+                }
+                cnt += pc;
             }
+            w.rem[i] = ki;
+            kb = min(kb, ki);
+        }
+        wsync();
+        // W_adj of the removals: full scans of N(r) for removals 1..kb-1,
+        // the partial scan up to the smallest failing member for removal kb
+        long long sw = 0;
+        for (int k = 1 + c.lane; k < kb; k += WAVE) sw += c.g->d1r[w.pid[t0 - k]];
+        wadj_add(c, wave_sum64(sw));
+        if (kb < bend) {
+            int fid = INT32_MAX;
+            for (int i = c.lane; i < nm; i += WAVE)
+                if (w.rem[i] == kb) fid = min(fid, w.pid[i]);
+            fid = uni(wave_min(fid));
+            const uint64_t* rrow = c.g->A1 + (size_t)w.pid[t0 - kb] * c.g->rw;
+            int rank = 0;
+            for (int wd = c.lane; wd <= (fid >> 6); wd += WAVE) {
+                uint64_t x = rrow[wd];
+                if (wd == (fid >> 6)) x &= (1ull << (fid & 63)) - 1ull;
+                rank += __popcll(x);
+            }
+            wadj_add(c, (long long)wave_sum(rank) + 1);
         }
         const int hi = min(kb, bend);
         const int ch = a.split_chunk;
@@ -1045,7 +1082,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         for (int b0 = bbeg; b0 < hi; b0 += ch) {
             if (c.lane == 0) {
                 const int top = a.n_buckets - 1;
-                unsigned long long ri = atomicAdd(&a.q_tail[top], 1ull);
+                unsigned long long ri = atomicAdd(&a.q_tail[top * QSTRIDE], 1ull);
                 if (ri >= a.q_cap) {
                     atomicOr(a.error, 1);
                 } else {
@@ -1232,34 +1269,46 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, a.slab_mwords);
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     constexpr int WAIT_LIMIT = 1 << 22;
+    int idle = 0;      // consecutive empty polls (back-off)
+    int sinst = -1;    // instance the wave's counters belong to
     for (;;) {
-        int state = 0;  // 0 idle, 1 got a slot, 2 done
+        int state = 0;  // 0 nothing published, 1 got a slot, 2 done, 3 lost a race
         unsigned long long h = 0;
         if (c.lane == 0) {
-            long long p = __hip_atomic_load(a.pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             int err = __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (p <= 0 || (err & 4)) {
-                // search complete, or a slot never got published: every
-                // wave leaves (the host reports the error)
-                state = 2;
+            if (err & 4) {
+                state = 2;  // a slot never got published: every wave leaves
             } else {
                 // highest non-empty bucket first
-                for (int b = a.n_buckets - 1; b >= 0 && state == 0; b--) {
-                    unsigned long long hb = __hip_atomic_load(&a.q_head[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    unsigned long long tb = __hip_atomic_load(&a.q_tail[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (hb < tb && hb < a.q_cap && atomicCAS(&a.q_head[b], hb, hb + 1) == hb) {
-                        state = 1;
-                        h = hb + (unsigned long long)b * a.q_cap;
+                for (int b = a.n_buckets - 1; b >= 0 && (state == 0 || state == 3); b--) {
+                    unsigned long long* hp = &a.q_head[b * QSTRIDE];
+                    unsigned long long hb = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    unsigned long long tb =
+                        __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (hb < tb && hb < a.q_cap) {
+                        if (atomicCAS(hp, hb, hb + 1) == hb) {
+                            state = 1;
+                            h = hb + (unsigned long long)b * a.q_cap;
+                        } else {
+                            state = 3;
+                        }
                     }
                 }
+                // every queue looked empty: finished once nothing is pending
+                // (a task counts itself out only after pushing its children)
+                if (state == 0 && __hip_atomic_load(a.pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+                    state = 2;
             }
         }
         state = uni(__shfl(state, 0, WAVE));
         if (state == 2) break;
+        if (state == 3) continue;
         if (state == 0) {
-            __builtin_amdgcn_s_sleep(8);
+            idle = idle < 16 ? idle + 1 : 16;
+            for (int k = 0; k < idle; k++) __builtin_amdgcn_s_sleep(32);
             continue;
         }
+        idle = 0;
         h = __shfl(h, 0, WAVE);
         unsigned int rd = 0;
         if (c.lane == 0) {
@@ -1278,6 +1327,10 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                 c.g = &a.graphs[ref.inst];
                 c.inst = ref.inst;
             }
+            if (a.inst_stats && (int)ref.inst != sinst) {
+                if (sinst >= 0) flush_stats(c, &a.inst_stats[sinst]);
+                sinst = ref.inst;
+            }
             c.slot = ref.b1 > 0 ? (long long)ref.bound : (long long)h;
             if (ref.t <= (uint32_t)TL) {
                 c.w = wl;
@@ -1291,12 +1344,14 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                 atomicOr(a.error, 2);
             }
             wsync();
-            if (a.inst_stats) flush_stats(c, &a.inst_stats[ref.inst]);
             __threadfence();
             if (c.lane == 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)-1ll);
         }
     }
-    flush_stats(c);
+    if (sinst >= 0)
+        flush_stats(c, &a.inst_stats[sinst]);
+    else
+        flush_stats(c);
 }
 
 }  // namespace qcdev

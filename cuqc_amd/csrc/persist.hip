@@ -22,7 +22,7 @@ __global__ void expand_persistent(const ExpandArgs*);
 // Queue state for a fresh run: roots in slots [0, nroots), their records at
 // the start of the pool, every other ready flag cleared, counters reset.
 // Roots go to the top bucket nb - 1; ctr = heads[nb], tails[nb], pool top,
-// pending.
+// pending, QSTRIDE words apart.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
                               uint64_t root_slots, TaskRef* q_refs, unsigned int* q_ready, uint64_t q_cap, int nb,
                               Vrec* pool, unsigned long long* ctr, StreamCtr* clq_ctr, DevStats* stats,
@@ -41,11 +41,11 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
     for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (tid == 0) {
         for (int b = 0; b < nb; b++) {
-            ctr[b] = 0;                                  // head
-            ctr[nb + b] = b == nb - 1 ? nroots : 0;      // tail
+            ctr[b * QSTRIDE] = 0;                                  // head
+            ctr[(nb + b) * QSTRIDE] = b == nb - 1 ? nroots : 0;    // tail
         }
-        ctr[2 * nb] = root_slots;      // pool top
-        ctr[2 * nb + 1] = nroots;      // pending
+        ctr[2 * nb * QSTRIDE] = root_slots;      // pool top
+        ctr[(2 * nb + 1) * QSTRIDE] = nroots;    // pending
         clq_ctr->items = 0;
         clq_ctr->slots = 0;
         *stats = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -166,7 +166,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     DBuf d_cmw, d_cmn;
     if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
     if ((rc = d_pool.alloc(pool_cap * sizeof(Vrec))) || (rc = d_q.alloc(nb * q_cap * sizeof(TaskRef))) ||
-        (rc = d_ready.alloc(nb * q_cap * 4)) || (rc = d_ctr.alloc(10 * 8)) ||
+        (rc = d_ready.alloc(nb * q_cap * 4)) || (rc = d_ctr.alloc(10 * QSTRIDE * 8)) ||
         (rc = d_slab.alloc((size_t)grid * slab_b)) ||
         (rc = d_clq_refs.alloc(clq_refs * sizeof(CliqueRef))) || (rc = d_clq_verts.alloc(clq_slots * 4)) ||
         (rc = d_clq_ctr.alloc(sizeof(StreamCtr))) || (rc = d_stats.alloc(sizeof(DevStats))) ||
@@ -204,9 +204,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     A.q_refs = d_q.as<TaskRef>();
     A.q_ready = d_ready.as<unsigned int>();
     A.q_head = d_ctr.as<unsigned long long>() + 0;
-    A.q_tail = d_ctr.as<unsigned long long>() + nb;
-    A.pool_top = d_ctr.as<unsigned long long>() + 2 * nb;
-    A.pending = (long long*)(d_ctr.as<unsigned long long>() + 2 * nb + 1);
+    A.q_tail = d_ctr.as<unsigned long long>() + nb * QSTRIDE;
+    A.pool_top = d_ctr.as<unsigned long long>() + 2 * nb * QSTRIDE;
+    A.pending = (long long*)(d_ctr.as<unsigned long long>() + (2 * nb + 1) * QSTRIDE);
     A.n_buckets = nb;
     A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
     A.cm_norm = cost ? d_cmn.as<float>() : nullptr;
@@ -260,7 +260,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         QC_HIP(hipEventElapsedTime(&ms, e0, e1));
         RunState rs{};
         std::vector<DevStats> ist(count);
-        QC_HIP(hipMemcpy(rs.ctr, d_ctr.p, sizeof(rs.ctr), hipMemcpyDeviceToHost));
+        {
+            unsigned long long raw[10 * QSTRIDE];
+            QC_HIP(hipMemcpy(raw, d_ctr.p, sizeof(raw), hipMemcpyDeviceToHost));
+            for (int k = 0; k < 10; k++) rs.ctr[k] = raw[k * QSTRIDE];
+        }
         QC_HIP(hipMemcpy(&rs.clq, d_clq_ctr.p, sizeof(rs.clq), hipMemcpyDeviceToHost));
         QC_HIP(hipMemcpy(&rs.err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
         QC_HIP(hipMemcpy(ist.data(), d_istats.p, (size_t)count * sizeof(DevStats), hipMemcpyDeviceToHost));

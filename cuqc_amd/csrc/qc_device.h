@@ -11,6 +11,9 @@
 // b1 > 0 marks a split item: branches [b0, b1) of a task whose record at
 // `off` was prepared by expand (label | lvl2adj << 4, exdeg before any
 // removal).
+// stride (in 64-bit words) between the persistent queue's shared counters
+constexpr int QSTRIDE = 32;
+
 struct TaskRef {
     uint64_t off;
     uint32_t t;
@@ -107,7 +110,8 @@ struct ExpandArgs {
 
     // persistent mode: n_buckets FIFO queues of task refs (bucket b owns
     // slots [b q_cap, (b+1) q_cap) of q_refs / q_ready and the counters
-    // q_head[b], q_tail[b]) over a bump-allocated vertex pool; q_ready[i]
+    // q_head[b QSTRIDE], q_tail[b QSTRIDE]; every shared counter sits on a
+    // 256-byte line of its own) over a bump-allocated vertex pool; q_ready[i]
     // publishes slot i; `pending` counts tasks pushed and not yet finished
     // (0 = search complete).  With a cost model, a child goes to the bucket
     // of its predicted log subtree size and waves pop the highest non-empty

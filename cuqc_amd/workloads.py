@@ -36,7 +36,7 @@ class Workload:
 
 WORKLOADS: dict[int, Workload] = {
     0: Workload("er200-planted", 0, 1, note="full search"),
-    1: Workload("coexpr8k", 1, 2, 0, -1, 2, note="all 4163 root branches, depth <= 2"),
+    1: Workload("coexpr8k", 1, 2, 0, -1, 1, note="all 4163 root branches, depth <= 1"),
     2: Workload("chunglu100k", 2, 3, note="window pending calibration"),
     3: Workload("batch256", 3, 4, note="per-instance windows"),
     4: Workload("chunglu1m", 4, 5, note="window pending calibration"),

@@ -80,13 +80,19 @@ def map_leak():
 
 
 def _window_part(args):
-    wl, inst, b0, b1 = args
+    """Counters of root branches [b0, b1) alone: the run over [b0, b1) minus
+    the run over the empty window [b0, b0), which holds the root's own work
+    (its task, lookahead and the removals that lead up to branch b0); the
+    first part of a window also returns that root share."""
+    wl, inst, b0, b1, first = args
     from cuqc_amd import workloads
     w = workloads.WORKLOADS[wl]
     g = workloads.graph(w, inst)
     off, adj = g.csr()
     r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=w.max_depth)
-    return r["stats"], [[int(x) for x in s] for s in r["raw"]]
+    e = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b0, max_depth=w.max_depth)
+    st = {k: r["stats"][k] - e["stats"][k] for k in STATS}
+    return st, [[int(x) for x in s] for s in r["raw"]], e["stats"] if first else None
 
 
 def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dict:
@@ -98,18 +104,19 @@ def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dic
     b0 = max(0, w.root_begin)
     b1 = nb if w.root_end < 0 else min(nb, w.root_end)
     cuts = sorted(set([b0] + [b0 + (b1 - b0) * i // parts for i in range(1, parts)] + [b1]))
-    tasks = [(wl, inst, a, b) for a, b in zip(cuts, cuts[1:])]
+    # the first part also returns the root's own counters
+    tasks = [(wl, inst, a, b, a == b0) for a, b in zip(cuts, cuts[1:])]
     tot = {k: 0 for k in STATS}
     raw = []
     with ProcessPoolExecutor(int(os.environ.get("GOLDEN_JOBS", jobs))) as ex:
-        for st, r in ex.map(_window_part, tasks):
+        for st, r, root in ex.map(_window_part, tasks):
             for k in STATS:
-                tot[k] += st[k] if k != "tasks" else st[k] - 1
+                tot[k] += st[k] + (root[k] if root is not None else 0)
             raw += r
-    tot["tasks"] += 1  # the root task, counted once
     mx = oracle.nonmax(raw)
     return {"workload": w.name, "instance": inst, "checksum": hex(g.checksum()), "window": list(w.window),
-            "root_branches": nb, "stats": tot, "maximal": len(mx), "maximal_sha256": digest(mx)}
+            "root_branches": nb, "stats": tot, "raw_sha256": digest(oracle.canon(raw)), "maximal": len(mx),
+            "maximal_sha256": digest(mx)}
 
 
 def main():

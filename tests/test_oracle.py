@@ -156,6 +156,15 @@ def test_window_partition(name, build, gamma, ms):
     assert oracle.canon(raw) == oracle.canon(full["raw"])
     # the root task is expanded once per window
     assert sum(p["stats"]["tasks"] for p in parts) - (len(parts) - 1) == full["stats"]["tasks"]
+    # every counter composes: root share (empty window at the first cut) plus
+    # each window's counters minus its empty window's, as the golden window
+    # fixtures are assembled
+    keys = ["tasks", "branches", "children", "raw_cliques", "w_adj", "sum_t", "sum_t_written", "sum_clique_len",
+            "beyond"]
+    empty = [oracle.mine(off, adj, gamma, ms, root_begin=a, root_end=a) for a in cut[:-1]]
+    for k in keys:
+        tot = empty[0]["stats"][k] + sum(p["stats"][k] - e["stats"][k] for p, e in zip(parts, empty))
+        assert tot == full["stats"][k], k
 
 
 def test_faithful_and_clean_map_modes():
