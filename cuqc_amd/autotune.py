@@ -1,0 +1,133 @@
+"""Schedule autotuner for the persistent miner.
+
+The reference fixes its launch shape and buffer sizes at compile time
+(main.cu:17-40: BLOCK_SIZE, TASKS_PER_WARP, the per-warp buffer sizes).  Here
+the schedule is a runtime choice and this module picks it per workload by
+timing candidate schedules on the workload itself (or on a sample window of
+it):
+
+* ``lds_tile``    tasks up to this many vertices run from LDS (64/128/256);
+* ``split_chunk`` root-range work items of this many branches for tasks above
+                  the LDS tile (-1: never split);
+* ``workgroups``  resident waves of the persistent launch (0: as many as fit).
+
+The search is coordinate descent from the library defaults, one parameter at
+a time, ``reps`` timed runs per candidate (median kernel time).  Every trial
+must reproduce the reference schedule's counters (tasks, branches, raw
+cliques, adjacency scans) and raw-set digest: a schedule changes the order of
+work, never the result, and a trial that disagrees raises.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+from dataclasses import asdict, dataclass, field
+
+SPACE: dict[str, list[int]] = {
+    "split_chunk": [4, 16, 64, -1],
+    "lds_tile": [64, 128, 256],
+    "workgroups": [512, 1024, 2048, 0],
+}
+DEFAULT = {"lds_tile": 128, "split_chunk": 16, "workgroups": 0}
+
+_INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
+
+
+@dataclass
+class Schedule:
+    lds_tile: int = 128
+    split_chunk: int = 16
+    workgroups: int = 0
+    kernel_ms: float = 0.0
+    trials: list = field(default_factory=list)
+
+    def kwargs(self) -> dict:
+        return {"lds_tile": self.lds_tile, "split_chunk": self.split_chunk, "workgroups": self.workgroups}
+
+
+def _digest(results) -> str:
+    h = hashlib.sha256()
+    for r in results:
+        for s in sorted(tuple(sorted(int(x) for x in s)) for s in r.raw):
+            h.update((" ".join(map(str, s)) + "\n").encode())
+        h.update(b"|")
+    return h.hexdigest()
+
+
+def _fingerprint(graphs, gammas, min_sizes, windows) -> str:
+    h = hashlib.sha256()
+    for g, ga, ms, w in zip(graphs, gammas, min_sizes, windows):
+        h.update(f"{g.checksum():x}:{ga!r}:{ms}:{tuple(w) if w is not None else None};".encode())
+    return h.hexdigest()[:32]
+
+
+def tune(graphs, gammas, min_sizes, windows=None, *, reps: int = 2, order: str = "fifo", cost_model=None,
+         cache: str | os.PathLike | None = None, log=None) -> Schedule:
+    """Pick the fastest schedule for mining ``graphs`` together (one batched
+    persistent launch; a single graph is a batch of one)."""
+    import numpy as np
+
+    from . import mine, mine_batch
+
+    n = len(graphs)
+    windows = list(windows) if windows is not None else [None] * n
+    key = _fingerprint(graphs, gammas, min_sizes, windows) + f":{order}"
+    if cache is not None and os.path.exists(cache):
+        with open(cache) as f:
+            known = json.load(f)
+        if key in known:
+            return Schedule(**known[key])
+
+    def run(cfg: dict):
+        common = dict(order=order, cost_model=cost_model, **cfg)
+        if n == 1:
+            return [mine(graphs[0], gammas[0], min_sizes[0], 0, mode="persistent", window=windows[0], **common)]
+        return mine_batch(graphs, gammas, min_sizes, windows=windows, **common)
+
+    ref_stats, ref_digest = None, None
+    seen: dict[tuple, float] = {}
+    trials = []
+
+    def measure(cfg: dict) -> float:
+        nonlocal ref_stats, ref_digest
+        k = (cfg["lds_tile"], cfg["split_chunk"], cfg["workgroups"])
+        if k in seen:
+            return seen[k]
+        times = []
+        for _ in range(reps):
+            res = run(cfg)
+            st = [{f: r.stats[f] for f in _INVARIANT} for r in res]
+            if ref_stats is None:
+                ref_stats, ref_digest = st, _digest(res)
+            elif st != ref_stats or _digest(res) != ref_digest:
+                raise RuntimeError(f"schedule {cfg} changed the mining result")
+            times.append(max(r.stats["t_kernel_ms"] for r in res))
+        t = float(np.median(times))
+        seen[k] = t
+        trials.append({**cfg, "kernel_ms": round(t, 3)})
+        if log:
+            log(f"[autotune] {cfg} -> {t:.2f} ms")
+        return t
+
+    best = dict(DEFAULT)
+    best_t = measure(best)
+    for name in ("split_chunk", "lds_tile", "workgroups"):
+        for v in SPACE[name]:
+            cand = dict(best, **{name: v})
+            t = measure(cand)
+            if t < best_t:
+                best, best_t = cand, t
+    sched = Schedule(kernel_ms=round(best_t, 3), trials=trials, **best)
+    if cache is not None:
+        known = {}
+        if os.path.exists(cache):
+            with open(cache) as f:
+                known = json.load(f)
+        known[key] = asdict(sched)
+        tmp = f"{cache}.{os.getpid()}.{time.time_ns()}"
+        with open(tmp, "w") as f:
+            json.dump(known, f, indent=1)
+        os.replace(tmp, cache)
+    return sched

@@ -127,3 +127,20 @@ def test_cost_order_requires_model_and_fifo_trace():
     m = qc.CostModel.create(0)
     with pytest.raises(qc.QCError):
         qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", order="cost", cost_model=m, trace=True)
+
+
+@pytest.mark.parametrize("kind", ["nan", "huge", "negated"])
+def test_cost_order_with_corrupted_model(cfg0_model, kind):
+    """The model only orders work: a corrupted one changes nothing."""
+    g, ref, model, _, _ = cfg0_model
+    p, nrm = model.params()
+    bad = qc.CostModel.create(seed=9)
+    if kind == "nan":
+        bad.set_params(np.full_like(p, np.nan), nrm)
+    elif kind == "huge":
+        bad.set_params(np.full_like(p, 3.0e38), nrm)
+    else:
+        bad.set_params(-p, nrm)
+    _say(f"[gpu] cost order corrupted model {kind}")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", order="cost", cost_model=bad, split_chunk=4)
+    _compare(res, ref)
