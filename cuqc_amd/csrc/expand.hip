@@ -175,7 +175,27 @@ struct Ctx {
     uint32_t cap;         // largest task the workspace holds
     uint32_t inst;        // instance of the task being expanded
     long long slot;       // queue slot of the task being expanded (trace)
+    unsigned long long ph[PH_COUNT];  // phase cycles (diagnostics)
+    long long ph_t;                   // start of the running phase
 };
+
+// phase accounting: charge the cycles since the last mark to phase k
+__device__ __forceinline__ void ph_mark(Ctx& c, int k) {
+    if (c.a->phase) {
+        long long n = clock64();
+        c.ph[k] += (unsigned long long)(n - c.ph_t);
+        c.ph_t = n;
+    }
+}
+__device__ __forceinline__ void ph_start(Ctx& c) {
+    for (int k = 0; k < PH_COUNT; k++) c.ph[k] = 0;
+    c.ph_t = clock64();
+}
+__device__ __forceinline__ void ph_flush(Ctx& c) {
+    if (c.a->phase && c.lane == 0)
+        for (int k = 0; k < PH_COUNT; k++)
+            if (c.ph[k]) atomicAdd(&c.a->phase[k], c.ph[k]);
+}
 
 __device__ __forceinline__ int md(const Ctx& c, int i) {
     // index stays below mindeg_n by construction of the table size
@@ -913,6 +933,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         c.M2 = c.g->A2;
     }
 #define PM(i) (LOCAL ? (i) : P2M[(i)])
+    ph_mark(c, PH_LOAD);
 
     // ---- lookahead, main.cu:1349-1405 ----
     int nc = t0 - nm;
@@ -939,10 +960,12 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             wsync();
             if (ballot(bad) == 0) {
                 emit_clique(c, w.pid, t0);
+                ph_mark(c, PH_LOOKAHEAD);
                 return;
             }
         }
     }
+    ph_mark(c, PH_LOOKAHEAD);
 
     // ---- branch loop, main.cu:1094-1180 ----
     // Branch ordinal b = j - covered.  Branch b >= 1 first removes the last
@@ -1010,6 +1033,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             tp = t0 - b0;
             nc -= b0;
         }
+        ph_mark(c, PH_REMOVE);
         b_lo = b0;
         b_hi = (int)ref.b1;
         count_removes = false;
@@ -1041,10 +1065,13 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         for (int i = 0; i < nm; i++) {
             const int need = max(w.pin[i] + w.pex[i] - mdg + 1, 1);
             const int row = PM(i);
+            // removals up to kb inclusive: a member failing at the same
+            // removal as an earlier one must be seen (smallest failing id)
+            const int lim = min(kb, bend - 1);
             int cnt = 0, ki = INT32_MAX;
-            for (int k0 = 1; k0 < kb && ki == INT32_MAX; k0 += WAVE) {
+            for (int k0 = 1; k0 <= lim && ki == INT32_MAX; k0 += WAVE) {
                 const int k = k0 + c.lane;
-                const bool nb_ = k < kb && gbit(c.M1, c.W, row, PM(t0 - k));
+                const bool nb_ = k <= lim && gbit(c.M1, c.W, row, PM(t0 - k));
                 const uint64_t bm = ballot(nb_);
                 const int pc = __popcll(bm);
                 if (cnt + pc >= need) {
@@ -1103,13 +1130,16 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
                 }
             }
         }
+        ph_mark(c, PH_PREPARE);
         return;
     }
 
     for (int b = b_lo; b < b_hi; b++) {
         if (!item && b >= bend) break;
         if (b > b_lo || (!item && b > 0)) {
-            if (remove_step(count_removes)) break;
+            const bool stop = remove_step(count_removes);
+            ph_mark(c, PH_REMOVE);
+            if (stop) break;
         }
         if (!item && b < bbeg) continue;
         // child record: members, branch vertex, remaining in order
@@ -1152,17 +1182,24 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             nind += __popcll(bo);
         }
         wsync();
+        ph_mark(c, PH_ADD);
         int ub = 0, lb = 0, med = 0;
         if (degree_pruning(c, tc, cc, m, ub, lb, med, nind, false)) {
             if (m >= c.g->min_size) check_for_clique(c, m);
+            ph_mark(c, PH_PRUNE);
             continue;
         }
         int r = critical_pruning(c, tc, cc, m, ub, lb, med);
-        if (r == 2) continue;
+        if (r == 2) {
+            ph_mark(c, PH_PRUNE);
+            continue;
+        }
         if (m >= c.g->min_size) check_for_clique(c, m);
+        ph_mark(c, PH_PRUNE);
         if (r == 1) continue;
         if (cc > 0) write_child(c, tc, m, ub - lb);
         wsync();
+        ph_mark(c, PH_CHILD);
     }
 #undef PM
 }
@@ -1231,6 +1268,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     c.w = make_ws(lds, TL, true, (TL + 63) / 64);
     c.cap = TL;
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ph_start(c);
     task_loop<true>(c);
 }
 
@@ -1246,6 +1284,7 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     c.w = make_ws(base, a.slab_tcap, false, a.slab_mwords);
     c.cap = a.slab_tcap;
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ph_start(c);
     task_loop<false>(c);
 }
 
@@ -1268,6 +1307,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     const WS wl = make_ws(lds, TL, true, (TL + 63) / 64);
     const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, a.slab_mwords);
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ph_start(c);
     constexpr int WAIT_LIMIT = 1 << 22;
     int idle = 0;      // consecutive empty polls (back-off)
     int sinst = -1;    // instance the wave's counters belong to
@@ -1321,6 +1361,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
             if (!rd) atomicOr(a.error, 4);
         }
         rd = (unsigned int)uni((int)__shfl(rd, 0, WAVE));
+        ph_mark(c, PH_IDLE);
         if (rd == 1) {
             const TaskRef ref = a.q_refs[h];
             if (a.graphs) {
@@ -1346,8 +1387,11 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
             wsync();
             __threadfence();
             if (c.lane == 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)-1ll);
+            ph_mark(c, PH_CHILD);
         }
     }
+    ph_mark(c, PH_IDLE);
+    ph_flush(c);
     if (sinst >= 0)
         flush_stats(c, &a.inst_stats[sinst]);
     else

@@ -222,6 +222,15 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const ExpandArgs* ap = d_args.as<ExpandArgs>();
+    // QCMINE_PHASES=1: per-phase cycle counters, printed to stderr
+    DBuf d_phase;
+    const bool phases = getenv("QCMINE_PHASES") && atoi(getenv("QCMINE_PHASES")) > 0;
+    if (phases) {
+        if ((rc = d_phase.alloc(PH_COUNT * 8))) return rc;
+        QC_HIP(hipMemset(d_phase.p, 0, PH_COUNT * 8));
+        A.phase = d_phase.as<unsigned long long>();
+        QC_HIP(hipMemcpy(d_args.p, &A, sizeof(A), hipMemcpyHostToDevice));
+    }
     void* kargs[] = {(void*)&ap};
     const uint32_t nroots = (uint32_t)roots.size();
     QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -284,6 +293,16 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
             set_error("graph replay %d differs from the first launch", rep);
             status = QC_EDEVICE;
         }
+    }
+    if (phases) {
+        unsigned long long ph[PH_COUNT];
+        QC_HIP(hipMemcpy(ph, d_phase.p, sizeof(ph), hipMemcpyDeviceToHost));
+        static const char* names[] = {"load", "lookahead", "remove", "add", "prune", "child", "prepare", "idle"};
+        double tot = 0;
+        for (int k = 0; k < PH_COUNT; k++) tot += (double)ph[k];
+        fprintf(stderr, "[qcmine] phases (grid %u waves, %.3g wave-cycles):", grid, tot);
+        for (int k = 0; k < PH_COUNT; k++) fprintf(stderr, " %s %.1f%%", names[k], 100.0 * ph[k] / (tot > 0 ? tot : 1));
+        fprintf(stderr, "\n");
     }
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);

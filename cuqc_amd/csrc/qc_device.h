@@ -142,4 +142,20 @@ struct ExpandArgs {
 
     DevStats* stats;
     int* error;  // bit 0: output overflow, bit 1: task above slab capacity, bit 2: queue wait limit
+
+    // diagnostics (NULL = off): shader cycles per phase summed over waves,
+    // indexed by the PH_* constants
+    unsigned long long* phase;
+};
+
+enum PhaseId {
+    PH_LOAD = 0,      // record load and local matrices
+    PH_LOOKAHEAD,     // lookahead
+    PH_REMOVE,        // remove_one_vertex steps and item catch-up
+    PH_ADD,           // add_one_vertex and diameter pruning
+    PH_PRUNE,         // degree / critical pruning and the clique test
+    PH_CHILD,         // child sort and write
+    PH_PREPARE,       // split preparation
+    PH_IDLE,          // persistent pop loop: waiting for work
+    PH_COUNT
 };
