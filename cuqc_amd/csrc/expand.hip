@@ -307,10 +307,72 @@ __device__ void sort_keys(Ctx& c, uint64_t* hi, uint64_t* lo, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// LU bounds, main.cu:1857-1972.  Members are child positions [0, m);
-// cind[0..nc) holds candidate indegrees sorted descending.
+// Candidate indegrees for the LU bounds.  The bounds only need P(k), the sum
+// of the k largest indegrees (main.cu:1919-1940 walks them sorted), and
+// indegrees lie in [0, m].  So instead of sorting, the n values in cind are
+// turned into suffix counts C[v] = #{x >= v} (in adjc) and suffix sums
+// S[v] = sum{x : x >= v} (in cind), v in [0, V + 1], V = the largest value;
+// then P(k) = S[v + 1] + (k - C[v + 1]) v with v the largest value whose
+// C[v] >= k.  Returns V, or -1 when the tables do not fit (the caller sorts).
 // ---------------------------------------------------------------------------
-__device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med) {
+__device__ int indeg_dist(Ctx& c, int n) {
+    WS& w = c.w;
+    int mx = 0;
+    for (int i = c.lane; i < n; i += WAVE) mx = max(mx, w.cind[i]);
+    const int V = uni(wave_max(mx));
+    if (V + 2 > w.T2) return -1;
+    int* C = w.adjc;
+    int* S = w.cind;
+    for (int v = c.lane; v <= V + 1; v += WAVE) C[v] = 0;
+    wsync();
+    for (int i = c.lane; i < n; i += WAVE) atomicAdd(&C[w.cind[i]], 1);
+    wsync();
+    // suffix scans from the top bin down: C <- counts >= v, S <- sums >= v
+    long long cc = 0, cs = 0;
+    for (int top = V; top >= 0; top -= WAVE) {
+        const int v = top - c.lane;  // lane 0 holds the highest bin of the chunk
+        const long long h = v >= 0 ? C[v] : 0;
+        const long long ic = wave_incl_scan64(h) + cc;
+        const long long is = wave_incl_scan64(h * (v >= 0 ? v : 0)) + cs;
+        cc = __shfl(ic, WAVE - 1, WAVE);
+        cs = __shfl(is, WAVE - 1, WAVE);
+        wsync();
+        if (v >= 0) {
+            C[v] = (int)ic;
+            S[v] = (int)is;
+        }
+        wsync();
+    }
+    if (c.lane == 0) {
+        C[V + 1] = 0;
+        S[V + 1] = 0;
+    }
+    wsync();
+    return V;
+}
+
+// P(k) from the distribution tables (k <= n)
+__device__ __forceinline__ long long top_sum(const WS& w, int V, int k) {
+    if (k <= 0) return 0;
+    const int* C = w.adjc;
+    const int* S = w.cind;
+    int lo = 0, hi = V;  // C[0] = n >= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (C[mid] >= k)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return (long long)S[lo + 1] + (long long)(k - C[lo + 1]) * lo;
+}
+
+// ---------------------------------------------------------------------------
+// LU bounds, main.cu:1857-1972.  Members are child positions [0, m).  With
+// V >= 0 the candidate indegrees are given by the tables of indeg_dist,
+// with V < 0 cind[0..nc) holds them sorted descending.
+// ---------------------------------------------------------------------------
+__device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int V) {
     WS& w = c.w;
     const ExpandArgs& a = *c.a;
     int min_in = INT32_MAX, min_tot = INT32_MAX;
@@ -361,9 +423,14 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med) {
             int first = -1, last = -1;
             for (int base = 0; base <= ub; base += WAVE) {
                 int idx = base + c.lane;  // P(idx) = prefix over [0, idx)
-                long long v = (idx - 1 >= 0 && idx - 1 < nc && idx <= ub) ? (long long)w.cind[idx - 1] : 0;
-                long long inc = wave_incl_scan64(v) + carry;  // = P(idx)
-                carry = __shfl(inc, WAVE - 1, WAVE);
+                long long inc;
+                if (V >= 0) {
+                    inc = idx <= ub ? top_sum(w, V, idx) : 0;
+                } else {
+                    long long v = (idx - 1 >= 0 && idx - 1 < nc && idx <= ub) ? (long long)w.cind[idx - 1] : 0;
+                    inc = wave_incl_scan64(v) + carry;  // = P(idx)
+                    carry = __shfl(inc, WAVE - 1, WAVE);
+                }
                 bool inr = idx <= ub;
                 bool cnd = inr && (sum_in + inc >= (long long)m * md(c, m + idx));
                 uint64_t bfirst = ballot(cnd && idx >= lb);
@@ -410,8 +477,9 @@ __device__ inline bool members_fail(Ctx& c, int m, int ub, int lb, int med) {
 __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb, int& med, int nind,
                                bool map_empty) {
     WS& w = c.w;
-    sort_desc_int(c, w.cind, nind);
-    if (lu_bounds(c, m, nind, ub, lb, med)) return true;
+    int V = indeg_dist(c, nind);
+    if (V < 0) sort_desc_int(c, w.cind, nind);
+    if (lu_bounds(c, m, nind, ub, lb, med, V)) return true;
     if (members_fail(c, m, ub, lb, med)) return true;
     // classify candidates, stable
     int nrem = 0, nrmv = 0;
@@ -468,8 +536,9 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
             nv += __popcll(bo);
         }
         wsync();
-        sort_desc_int(c, w.cind, nv);
-        if (lu_bounds(c, m, nv, ub, lb, med)) return true;
+        V = indeg_dist(c, nv);
+        if (V < 0) sort_desc_int(c, w.cind, nv);
+        if (lu_bounds(c, m, nv, ub, lb, med, V)) return true;
         if (members_fail(c, m, ub, lb, med)) return true;
         // re-classify the remaining set under the new bounds (in place)
         int kept = 0;
@@ -863,7 +932,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
     if (!item) {  // a split item's task is counted by its prepare step
         c.st.tasks++;
         c.st.sum_t += t0;
-        if (!LOCAL) c.st.large++;
+        if ((uint32_t)t0 > c.a->small_max) c.st.large++;
     }
 
     for (int i = c.lane; i < t0; i += WAVE) {
@@ -1037,8 +1106,7 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
         b_lo = b0;
         b_hi = (int)ref.b1;
         count_removes = false;
-    } else if (!LOCAL && a.queue_mode && a.split_chunk > 0 && (uint32_t)t0 > a.split_min_t &&
-               bend - bbeg > a.split_chunk) {
+    } else if (a.queue_mode && a.split_chunk > 0 && (uint32_t)t0 > a.split_min_t && bend - bbeg > a.split_chunk) {
         // prepare step: keep the record (with lvl2adj) for the items, find
         // the first failing removal, count the removals' scans, push items
         unsigned long long slot = 0;
@@ -1305,7 +1373,10 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     c.slot = -1;
     c.lane = lane_id();
     const WS wl = make_ws(lds, TL, true, (TL + 63) / 64);
-    const WS wg = make_ws(a.slab + (size_t)blockIdx.x * a.slab_bytes, a.slab_tcap, false, a.slab_mwords);
+    char* const slab = a.slab + (size_t)blockIdx.x * a.slab_bytes;
+    const WS wg = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
+    const uint32_t mid = a.mid_tcap;
+    const WS wm = make_ws(slab, mid, true, (mid + 63) / 64);
     c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     constexpr int WAIT_LIMIT = 1 << 22;
@@ -1376,6 +1447,10 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
             if (ref.t <= (uint32_t)TL) {
                 c.w = wl;
                 c.cap = TL;
+                process_task<true>(c, ref);
+            } else if (ref.t <= mid) {
+                c.w = wm;
+                c.cap = mid;
                 process_task<true>(c, ref);
             } else if (ref.t <= a.slab_tcap) {
                 c.w = wg;

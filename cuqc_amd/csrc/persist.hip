@@ -11,7 +11,10 @@
 // replayed.
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "qc_internal.h"
 
@@ -57,12 +60,18 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
 namespace qc {
 
 namespace {
-size_t slab_bytes(uint32_t tcap, uint32_t rw) {
+
+// Per-wave HBM slab: the root-space workspace for tasks above the mid size
+// and, in the same bytes, the local-matrix workspace for tasks between the
+// LDS tile and the mid size (a wave runs one task at a time).
+size_t ws_bytes_host(uint32_t tcap, bool mat, uint32_t mwords) {
     uint32_t T2 = 64;
     while (T2 < tcap) T2 <<= 1;
+    const uint32_t W = (tcap + 63) / 64;
     size_t b = (size_t)15 * T2 * 4;
     b = (b + 15) & ~(size_t)15;
-    b += (size_t)2 * rw * 8;
+    if (mat) b += (size_t)2 * tcap * W * 8;
+    b += (size_t)2 * mwords * 8;
     return (b + 255) & ~(size_t)255;
 }
 
@@ -71,6 +80,32 @@ struct RunState {
     StreamCtr clq;
     int err;
 };
+
+// Device buffers of the persistent form, kept per device between calls (grow
+// only), so a steady stream of calls allocates nothing.  Calls on one device
+// serialize on the mutex.  Never destroyed: the runtime may be gone at exit.
+struct Workspace {
+    std::mutex mu;
+    DBuf pool, q, ready, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args;
+    uint64_t pool_cap = 0, q_cap = 0, clq_slots = 0, clq_refs_cap = 0;
+
+    static Workspace& get(int dev) {
+        static std::mutex m;
+        static std::vector<Workspace*> all;
+        std::lock_guard<std::mutex> g(m);
+        if ((int)all.size() <= dev) all.resize(dev + 1, nullptr);
+        if (!all[dev]) all[dev] = new Workspace;
+        return *all[dev];
+    }
+};
+
+int ensure(DBuf& b, size_t bytes) {
+    if (b.p && b.bytes >= bytes) return QC_OK;
+    return b.alloc(bytes);
+}
+
+// tasks up to this many vertices use task-local matrices in the slab
+constexpr int MID_MAX = 1024;
 }  // namespace
 
 int enumerate_persistent(const Instance* const* insts, const qc_params* params, int count, EnumOut* outs,
@@ -133,166 +168,243 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     QC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
     // every wave must be resident: the queue hand-off waits on producers
-    uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, per_cu);
+    uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 8));
     if (p.workgroups > 0) grid = std::min<uint32_t>(grid, (uint32_t)p.workgroups);
 
+    const uint32_t mid = (uint32_t)std::min(maxR, MID_MAX);
+    const uint32_t midw = (mid + 63) / 64;
+    const bool use_mid = (int)mid > TL;
+    const size_t slab_b = std::max(ws_bytes_host((uint32_t)maxR, false, (uint32_t)maxrw),
+                                   use_mid ? ws_bytes_host(mid, true, midw) : (size_t)0);
     size_t freeb = 0, totalb = 0;
     QC_HIP(hipMemGetInfo(&freeb, &totalb));
-    const size_t slab_b = slab_bytes((uint32_t)maxR, (uint32_t)maxrw);
-    const size_t budget = std::min<size_t>(freeb / 2, (size_t)128 << 30);
-    if ((size_t)grid * slab_b > budget / 4) grid = (uint32_t)std::max<size_t>(1, budget / 4 / slab_b);
-    // capacities: the FIFO keeps every record it held (no reuse), so they
-    // bound the tasks / vertex slots / raw cliques of one call
-    const uint64_t pool_cap = std::max<uint64_t>(
-        root_verts.size() + 1024,
-        p.task_capacity ? p.task_capacity : std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec));
-    // per bucket
-    const uint64_t q_cap =
-        std::max<uint64_t>(roots.size() + 1024, std::min<uint64_t>(pool_cap / 8 + 1024, (1ull << 26) / nb));
-    const uint64_t clq_slots =
-        p.clique_capacity ? p.clique_capacity : std::min<uint64_t>(budget / 16 / 4, (uint64_t)1 << 30);
-    const uint64_t clq_refs = std::min<uint64_t>(clq_slots / 4 + 1024, (uint64_t)1 << 26);
 
-    DBuf d_pool, d_q, d_ready, d_ctr, d_slab, d_clq_refs, d_clq_verts, d_clq_ctr, d_stats, d_istats, d_err, d_args,
-        d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_rfeat;
-    const bool trace = p.trace != 0;
-    if (trace) {
-        if ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8)) ||
-            (rc = d_rfeat.alloc(root_feat.size() * sizeof(float))))
-            return rc;
-        QC_HIP(hipMemcpyAsync(d_rfeat.p, root_feat.data(), root_feat.size() * sizeof(float), hipMemcpyHostToDevice,
-                              s));
+    Workspace& ws = Workspace::get(dev);
+    std::lock_guard<std::mutex> lock(ws.mu);
+    const size_t budget = std::min<size_t>((freeb + ws.pool.bytes + ws.q.bytes + ws.ready.bytes + ws.slab.bytes +
+                                            ws.clq_refs.bytes + ws.clq_verts.bytes) / 2,
+                                           (size_t)128 << 30);
+    if ((size_t)grid * slab_b > budget / 4) grid = (uint32_t)std::max<size_t>(1, budget / 4 / slab_b);
+
+    // capacities: the FIFO keeps every record it held (no reuse); they start
+    // from what the device workspace already holds and grow by rerunning the
+    // (deterministic) launch when a run overflows, unless the caller fixed them
+    const bool fixed_tasks = p.task_capacity != 0, fixed_clq = p.clique_capacity != 0;
+    const uint64_t max_pool = std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec);
+    const uint64_t max_q = (1ull << 28) / nb;
+    const uint64_t max_clq = std::min<uint64_t>(budget / 16 / 4, (uint64_t)1 << 31);
+    uint64_t pool_cap = fixed_tasks ? p.task_capacity
+                                    : std::max<uint64_t>(ws.pool_cap, std::min<uint64_t>(max_pool, 1ull << 26));
+    pool_cap = std::max<uint64_t>(pool_cap, root_verts.size() + 1024);
+    uint64_t q_cap = std::max<uint64_t>(ws.q_cap / nb, std::min<uint64_t>(max_q, 1ull << 22));
+    q_cap = std::max<uint64_t>(q_cap, roots.size() + 1024);
+    uint64_t clq_slots = fixed_clq ? p.clique_capacity
+                                   : std::max<uint64_t>(ws.clq_slots, std::min<uint64_t>(max_clq, 1ull << 24));
+    uint64_t clq_refs = std::max<uint64_t>(ws.clq_refs_cap, clq_slots / 8 + 1024);
+    if (const char* ic = getenv("QCMINE_INITIAL_CAPACITY")) {
+        // start small (tests of the grow-and-rerun path)
+        const uint64_t c0 = std::max<uint64_t>(1, strtoull(ic, nullptr, 10));
+        if (!fixed_tasks) pool_cap = std::max<uint64_t>(c0, root_verts.size() + 1024);
+        q_cap = std::max<uint64_t>(c0, roots.size() + 1024);
+        if (!fixed_clq) clq_slots = c0;
+        clq_refs = c0 / 8 + 1024;
     }
-    DBuf d_cmw, d_cmn;
+
+    const bool trace = p.trace != 0;
+    DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_rfeat, d_cmw, d_cmn, d_phase;
     if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
-    if ((rc = d_pool.alloc(pool_cap * sizeof(Vrec))) || (rc = d_q.alloc(nb * q_cap * sizeof(TaskRef))) ||
-        (rc = d_ready.alloc(nb * q_cap * 4)) || (rc = d_ctr.alloc(10 * QSTRIDE * 8)) ||
-        (rc = d_slab.alloc((size_t)grid * slab_b)) ||
-        (rc = d_clq_refs.alloc(clq_refs * sizeof(CliqueRef))) || (rc = d_clq_verts.alloc(clq_slots * 4)) ||
-        (rc = d_clq_ctr.alloc(sizeof(StreamCtr))) || (rc = d_stats.alloc(sizeof(DevStats))) ||
-        (rc = d_istats.alloc((size_t)count * sizeof(DevStats))) || (rc = d_err.alloc(sizeof(int))) ||
-        (rc = d_args.alloc(sizeof(ExpandArgs))) || (rc = d_graphs.alloc((size_t)count * sizeof(DevGraph))) ||
+    if ((rc = ensure(ws.ctr, 10 * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
+        (rc = ensure(ws.stats, sizeof(DevStats))) || (rc = ensure(ws.err, sizeof(int))) ||
+        (rc = ensure(ws.args, sizeof(ExpandArgs))) || (rc = ensure(ws.slab, (size_t)grid * slab_b)) ||
+        (rc = d_istats.alloc((size_t)count * sizeof(DevStats))) ||
+        (rc = d_graphs.alloc((size_t)count * sizeof(DevGraph))) ||
         (rc = d_roots.alloc(roots.size() * sizeof(TaskRef))) ||
         (rc = d_root_verts.alloc(root_verts.size() * sizeof(Vrec))))
         return rc;
+    if (trace && (rc = d_rfeat.alloc(root_feat.size() * sizeof(float)))) return rc;
     QC_HIP(hipMemcpyAsync(d_graphs.p, graphs.data(), (size_t)count * sizeof(DevGraph), hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(d_roots.p, roots.data(), roots.size() * sizeof(TaskRef), hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(d_root_verts.p, root_verts.data(), root_verts.size() * sizeof(Vrec), hipMemcpyHostToDevice,
                           s));
-
-    ExpandArgs A;
-    memset(&A, 0, sizeof(A));
-    A.g = graphs[0];
-    A.graphs = d_graphs.as<DevGraph>();
-    A.inst_stats = d_istats.as<DevStats>();
-    A.in_verts = d_pool.as<Vrec>();
-    A.small_max = (uint32_t)TL;
-    A.clq_refs = d_clq_refs.as<CliqueRef>();
-    A.clq_verts = d_clq_verts.as<uint32_t>();
-    A.clq_ctr = d_clq_ctr.as<StreamCtr>();
-    A.clq_ref_cap = clq_refs;
-    A.clq_vert_cap = clq_slots;
-    A.slab = d_slab.as<char>();
-    A.slab_bytes = slab_b;
-    A.slab_tcap = (uint32_t)maxR;
-    A.slab_mwords = (uint32_t)maxrw;
-    A.stats = d_stats.as<DevStats>();
-    A.error = d_err.as<int>();
-    A.queue_mode = 1;
-    A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 16 : p.split_chunk);
-    A.split_min_t = (uint32_t)TL;
-    A.q_refs = d_q.as<TaskRef>();
-    A.q_ready = d_ready.as<unsigned int>();
-    A.q_head = d_ctr.as<unsigned long long>() + 0;
-    A.q_tail = d_ctr.as<unsigned long long>() + nb * QSTRIDE;
-    A.pool_top = d_ctr.as<unsigned long long>() + 2 * nb * QSTRIDE;
-    A.pending = (long long*)(d_ctr.as<unsigned long long>() + (2 * nb + 1) * QSTRIDE);
-    A.n_buckets = nb;
-    A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
-    A.cm_norm = cost ? d_cmn.as<float>() : nullptr;
-    A.q_cap = q_cap;
-    A.pool = d_pool.as<Vrec>();
-    A.pool_cap = pool_cap;
-    A.trace_feat = trace ? d_tfeat.as<float>() : nullptr;
-    A.trace_parent = trace ? (long long*)d_tpar.p : nullptr;
-    QC_HIP(hipMemcpyAsync(d_args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
-    QC_HIP(hipStreamSynchronize(s));
-
-    // capture reset + expansion
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    const ExpandArgs* ap = d_args.as<ExpandArgs>();
+    if (trace)
+        QC_HIP(hipMemcpyAsync(d_rfeat.p, root_feat.data(), root_feat.size() * sizeof(float), hipMemcpyHostToDevice,
+                              s));
     // QCMINE_PHASES=1: per-phase cycle counters, printed to stderr
-    DBuf d_phase;
     const bool phases = getenv("QCMINE_PHASES") && atoi(getenv("QCMINE_PHASES")) > 0;
     if (phases) {
         if ((rc = d_phase.alloc(PH_COUNT * 8))) return rc;
-        QC_HIP(hipMemset(d_phase.p, 0, PH_COUNT * 8));
-        A.phase = d_phase.as<unsigned long long>();
-        QC_HIP(hipMemcpy(d_args.p, &A, sizeof(A), hipMemcpyHostToDevice));
-    }
-    void* kargs[] = {(void*)&ap};
-    const uint32_t nroots = (uint32_t)roots.size();
-    QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
-                       d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), d_q.as<TaskRef>(),
-                       d_ready.as<unsigned int>(), q_cap, nb, d_pool.as<Vrec>(), d_ctr.as<unsigned long long>(),
-                       d_clq_ctr.as<StreamCtr>(), d_stats.as<DevStats>(), d_istats.as<DevStats>(),
-                       (uint32_t)count, d_err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent);
-    hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, 0, s);
-    hipError_t ce = hipStreamEndCapture(s, &graph);
-    if (le != hipSuccess || ce != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        set_error("graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
-        return QC_EDEVICE;
-    }
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        set_error("hipGraphInstantiate failed");
-        return QC_EDEVICE;
+        QC_HIP(hipMemsetAsync(d_phase.p, 0, PH_COUNT * 8, s));
     }
 
     hipEvent_t e0, e1;
     QC_HIP(hipEventCreate(&e0));
     QC_HIP(hipEventCreate(&e1));
-    const int replays = std::max(1, p.replays);
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } eg{e0, e1};
+
     RunState first{};
     std::vector<DevStats> first_stats;
     float first_ms = 0;
-    int status = QC_OK;
-    for (int rep = 0; rep < replays && status == QC_OK; rep++) {
-        QC_HIP(hipEventRecord(e0, s));
-        QC_HIP(hipGraphLaunch(exec, s));
-        QC_HIP(hipEventRecord(e1, s));
-        QC_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        QC_HIP(hipEventElapsedTime(&ms, e0, e1));
-        RunState rs{};
-        std::vector<DevStats> ist(count);
-        {
-            unsigned long long raw[10 * QSTRIDE];
-            QC_HIP(hipMemcpy(raw, d_ctr.p, sizeof(raw), hipMemcpyDeviceToHost));
-            for (int k = 0; k < 10; k++) rs.ctr[k] = raw[k * QSTRIDE];
+    for (int attempt = 0;; attempt++) {
+        if ((rc = ensure(ws.pool, pool_cap * sizeof(Vrec))) || (rc = ensure(ws.q, nb * q_cap * sizeof(TaskRef))) ||
+            (rc = ensure(ws.ready, nb * q_cap * 4)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
+            (rc = ensure(ws.clq_verts, clq_slots * 4)))
+            return rc;
+        if (trace && ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8))))
+            return rc;
+        ws.pool_cap = std::max(ws.pool_cap, pool_cap);
+        ws.q_cap = std::max<uint64_t>(ws.q_cap, nb * q_cap);
+        ws.clq_slots = std::max(ws.clq_slots, clq_slots);
+        ws.clq_refs_cap = std::max(ws.clq_refs_cap, clq_refs);
+
+        ExpandArgs A;
+        memset(&A, 0, sizeof(A));
+        A.g = graphs[0];
+        A.graphs = d_graphs.as<DevGraph>();
+        A.inst_stats = d_istats.as<DevStats>();
+        A.in_verts = ws.pool.as<Vrec>();
+        A.small_max = (uint32_t)TL;
+        A.clq_refs = ws.clq_refs.as<CliqueRef>();
+        A.clq_verts = ws.clq_verts.as<uint32_t>();
+        A.clq_ctr = ws.clq_ctr.as<StreamCtr>();
+        A.clq_ref_cap = clq_refs;
+        A.clq_vert_cap = clq_slots;
+        A.slab = ws.slab.as<char>();
+        A.slab_bytes = slab_b;
+        A.slab_tcap = (uint32_t)maxR;
+        A.slab_mwords = (uint32_t)maxrw;
+        A.mid_tcap = use_mid ? mid : 0;
+        A.stats = ws.stats.as<DevStats>();
+        A.error = ws.err.as<int>();
+        A.queue_mode = 1;
+        A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 16 : p.split_chunk);
+        A.split_min_t = (uint32_t)TL;
+        A.q_refs = ws.q.as<TaskRef>();
+        A.q_ready = ws.ready.as<unsigned int>();
+        unsigned long long* ctr = ws.ctr.as<unsigned long long>();
+        A.q_head = ctr;
+        A.q_tail = ctr + nb * QSTRIDE;
+        A.pool_top = ctr + 2 * nb * QSTRIDE;
+        A.pending = (long long*)(ctr + (2 * nb + 1) * QSTRIDE);
+        A.n_buckets = nb;
+        A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
+        A.cm_norm = cost ? d_cmn.as<float>() : nullptr;
+        A.q_cap = q_cap;
+        A.pool = ws.pool.as<Vrec>();
+        A.pool_cap = pool_cap;
+        A.trace_feat = trace ? d_tfeat.as<float>() : nullptr;
+        A.trace_parent = trace ? (long long*)d_tpar.p : nullptr;
+        A.phase = phases ? d_phase.as<unsigned long long>() : nullptr;
+        QC_HIP(hipMemcpyAsync(ws.args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
+        QC_HIP(hipStreamSynchronize(s));
+
+        // capture reset + expansion
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const ExpandArgs* ap = ws.args.as<ExpandArgs>();
+        void* kargs[] = {(void*)&ap};
+        const uint32_t nroots = (uint32_t)roots.size();
+        QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
+                           d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), ws.q.as<TaskRef>(),
+                           ws.ready.as<unsigned int>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
+                           ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
+                           (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent);
+        hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, 0, s);
+        hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            set_error("graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+            return QC_EDEVICE;
         }
-        QC_HIP(hipMemcpy(&rs.clq, d_clq_ctr.p, sizeof(rs.clq), hipMemcpyDeviceToHost));
-        QC_HIP(hipMemcpy(&rs.err, d_err.p, sizeof(int), hipMemcpyDeviceToHost));
-        QC_HIP(hipMemcpy(ist.data(), d_istats.p, (size_t)count * sizeof(DevStats), hipMemcpyDeviceToHost));
-        if (rs.err) {
-            set_error("persistent expansion failed (error flags %d: 1 capacity, 2 task too large, 4 queue hand-off)",
-                      rs.err);
-            status = (rs.err & 4) ? QC_EDEVICE : QC_EOVERFLOW;
-            break;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            set_error("hipGraphInstantiate failed");
+            return QC_EDEVICE;
         }
-        if (rep == 0) {
-            first = rs;
-            first_stats = ist;
-            first_ms = ms;
-        } else if (memcmp(first.ctr, rs.ctr, sizeof(rs.ctr)) != 0 || first.clq.items != rs.clq.items ||
-                   first.clq.slots != rs.clq.slots ||
-                   memcmp(first_stats.data(), ist.data(), (size_t)count * sizeof(DevStats)) != 0) {
-            set_error("graph replay %d differs from the first launch", rep);
-            status = QC_EDEVICE;
+        struct GraphGuard {
+            hipGraph_t g;
+            hipGraphExec_t e;
+            ~GraphGuard() {
+                (void)hipGraphExecDestroy(e);
+                (void)hipGraphDestroy(g);
+            }
+        } gg{graph, exec};
+
+        const int replays = std::max(1, p.replays);
+        int status = QC_OK;
+        bool overflow = false;
+        for (int rep = 0; rep < replays && status == QC_OK; rep++) {
+            QC_HIP(hipEventRecord(e0, s));
+            QC_HIP(hipGraphLaunch(exec, s));
+            QC_HIP(hipEventRecord(e1, s));
+            QC_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            QC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            RunState rs{};
+            std::vector<DevStats> ist(count);
+            {
+                unsigned long long raw[10 * QSTRIDE];
+                QC_HIP(hipMemcpy(raw, ctr, sizeof(raw), hipMemcpyDeviceToHost));
+                for (int k = 0; k < 10; k++) rs.ctr[k] = raw[k * QSTRIDE];
+            }
+            QC_HIP(hipMemcpy(&rs.clq, ws.clq_ctr.p, sizeof(rs.clq), hipMemcpyDeviceToHost));
+            QC_HIP(hipMemcpy(&rs.err, ws.err.p, sizeof(int), hipMemcpyDeviceToHost));
+            QC_HIP(hipMemcpy(ist.data(), d_istats.p, (size_t)count * sizeof(DevStats), hipMemcpyDeviceToHost));
+            if (rs.err) {
+                if (rs.err == 1 && rep == 0) {
+                    // grow what ran out (its counter kept counting) and rerun
+                    const uint64_t pool_need = rs.ctr[2 * nb];
+                    uint64_t q_need = 0;
+                    for (int b = 0; b < nb; b++) q_need = std::max<uint64_t>(q_need, rs.ctr[nb + b]);
+                    bool grown = false;
+                    if (pool_need > pool_cap && !fixed_tasks && pool_cap < max_pool) {
+                        pool_cap = std::min<uint64_t>(max_pool, std::max<uint64_t>(pool_cap * 4, pool_need * 2));
+                        grown = true;
+                    }
+                    if (q_need > q_cap && q_cap < max_q) {
+                        q_cap = std::min<uint64_t>(max_q, std::max<uint64_t>(q_cap * 4, q_need * 2));
+                        grown = true;
+                    }
+                    if ((rs.clq.slots > clq_slots || rs.clq.items > clq_refs) && !fixed_clq && clq_slots < max_clq) {
+                        clq_slots = std::min<uint64_t>(max_clq, std::max<uint64_t>(clq_slots * 4, rs.clq.slots * 2));
+                        clq_refs = std::max<uint64_t>(clq_refs * 4, rs.clq.items * 2);
+                        grown = true;
+                    }
+                    if (grown) {
+                        overflow = true;
+                        break;
+                    }
+                }
+                set_error("persistent expansion failed (error flags %d: 1 capacity, 2 task too large, 4 queue "
+                          "hand-off)",
+                          rs.err);
+                status = (rs.err & 4) ? QC_EDEVICE : QC_EOVERFLOW;
+                break;
+            }
+            if (rep == 0) {
+                first = rs;
+                first_stats = ist;
+                first_ms = ms;
+            } else if (memcmp(first.ctr, rs.ctr, sizeof(rs.ctr)) != 0 || first.clq.items != rs.clq.items ||
+                       first.clq.slots != rs.clq.slots ||
+                       memcmp(first_stats.data(), ist.data(), (size_t)count * sizeof(DevStats)) != 0) {
+                set_error("graph replay %d differs from the first launch", rep);
+                status = QC_EDEVICE;
+            }
         }
+        if (status != QC_OK) return status;
+        if (overflow && attempt < 8) continue;
+        if (overflow) {
+            set_error("persistent expansion: capacities still exceeded after growing");
+            return QC_EOVERFLOW;
+        }
+        break;
     }
     if (phases) {
         unsigned long long ph[PH_COUNT];
@@ -301,22 +413,18 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         double tot = 0;
         for (int k = 0; k < PH_COUNT; k++) tot += (double)ph[k];
         fprintf(stderr, "[qcmine] phases (grid %u waves, %.3g wave-cycles):", grid, tot);
-        for (int k = 0; k < PH_COUNT; k++) fprintf(stderr, " %s %.1f%%", names[k], 100.0 * ph[k] / (tot > 0 ? tot : 1));
+        for (int k = 0; k < PH_COUNT; k++)
+            fprintf(stderr, " %s %.1f%%", names[k], 100.0 * ph[k] / (tot > 0 ? tot : 1));
         fprintf(stderr, "\n");
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (status != QC_OK) return status;
 
     // per-instance outputs (the last replay's buffers; replays are identical
     // in every counter, the raw sets only in their order)
     StreamCtr cc = first.clq;
     std::vector<CliqueRef> hr(cc.items);
     std::vector<uint32_t> hv(cc.slots);
-    if (cc.items) QC_HIP(hipMemcpy(hr.data(), d_clq_refs.p, cc.items * sizeof(CliqueRef), hipMemcpyDeviceToHost));
-    if (cc.slots) QC_HIP(hipMemcpy(hv.data(), d_clq_verts.p, cc.slots * 4, hipMemcpyDeviceToHost));
+    if (cc.items) QC_HIP(hipMemcpy(hr.data(), ws.clq_refs.p, cc.items * sizeof(CliqueRef), hipMemcpyDeviceToHost));
+    if (cc.slots) QC_HIP(hipMemcpy(hv.data(), ws.clq_verts.p, cc.slots * 4, hipMemcpyDeviceToHost));
     for (auto& r : hr) {
         EnumOut& o = outs[r.inst];
         o.raw_v.insert(o.raw_v.end(), hv.begin() + r.off, hv.begin() + r.off + r.len);
@@ -337,7 +445,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         std::vector<TaskRef> qr(tail);
         QC_HIP(hipMemcpy(tf.data(), d_tfeat.p, tail * 12 * sizeof(float), hipMemcpyDeviceToHost));
         QC_HIP(hipMemcpy(tp.data(), d_tpar.p, tail * 8, hipMemcpyDeviceToHost));
-        QC_HIP(hipMemcpy(qr.data(), d_q.p, tail * sizeof(TaskRef), hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(qr.data(), ws.q.p, tail * sizeof(TaskRef), hipMemcpyDeviceToHost));
         std::vector<int64_t> sub(tail, 1), row(tail, -1);
         for (uint64_t i = tail; i-- > 0;)
             if (tp[i] >= 0) sub[tp[i]] += sub[i];

@@ -107,6 +107,9 @@ struct ExpandArgs {
     unsigned long long slab_bytes;   // per wave
     uint32_t slab_tcap;              // vertices per slab
     uint32_t slab_mwords;            // mask words per slab (root-space row words)
+    uint32_t mid_tcap;               // persistent mode: tasks above the LDS tile and up to
+                                     // this size use task-local matrices in the slab (0: none)
+    uint32_t pad3;
 
     // persistent mode: n_buckets FIFO queues of task refs (bucket b owns
     // slots [b q_cap, (b+1) q_cap) of q_refs / q_ready and the counters

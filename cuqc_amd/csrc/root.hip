@@ -13,6 +13,9 @@
 // Q-order key, main.cu:2098-2101).
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "qc_internal.h"
@@ -81,6 +84,67 @@ __global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off
         for (uint64_t j = b0; j < b1; j++) {
             uint32_t u = adj[j];
             for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) vis[adj[k] >> 6] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// The same rows and counts with the visited bitmap and both rows in LDS
+// (dynamic: ceil(n/64) + 2 rw words), for graphs whose bitmap fits.  Rows are
+// written whole, so A1 / A2 need no clearing.
+__global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                    const int32_t* __restrict__ loc, const uint32_t* __restrict__ orig,
+                                                    int32_t R, int32_t rw, int32_t n, uint64_t* A1, uint64_t* A2,
+                                                    int32_t* n2out) {
+    extern __shared__ unsigned long long sh[];
+    const int lane = threadIdx.x & 63;
+    const int nw = (n + 63) / 64;
+    unsigned long long* vis = sh;
+    unsigned long long* r1 = sh + nw;
+    unsigned long long* r2 = r1 + rw;
+    for (int a = blockIdx.x; a < R; a += gridDim.x) {
+        const uint32_t v = orig[a];
+        for (int w = lane; w < nw; w += 64) vis[w] = 0;
+        for (int w = lane; w < rw; w += 64) {
+            r1[w] = 0;
+            r2[w] = 0;
+        }
+        __syncthreads();
+        if (lane == 0) atomicOr(&vis[v >> 6], 1ull << (v & 63));
+        __syncthreads();
+        const uint64_t b0 = off[v], b1 = off[v + 1];
+        int cnt = 0;
+        for (uint64_t j = b0 + lane; j < b1; j += 64) {
+            const uint32_t u = adj[j];
+            const unsigned long long bit = 1ull << (u & 63);
+            const unsigned long long old = atomicOr(&vis[u >> 6], bit);
+            const int la = loc[u];
+            if (la >= 0) atomicOr(&r1[la >> 6], 1ull << (la & 63));
+            if (!(old & bit)) {
+                cnt++;
+                if (la >= 0) atomicOr(&r2[la >> 6], 1ull << (la & 63));
+            }
+        }
+        __syncthreads();
+        for (uint64_t j = b0; j < b1; j++) {
+            const uint32_t u = adj[j];
+            for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) {
+                const uint32_t x = adj[k];
+                const unsigned long long bit = 1ull << (x & 63);
+                const unsigned long long old = atomicOr(&vis[x >> 6], bit);
+                if (!(old & bit)) {
+                    cnt++;
+                    const int la = loc[x];
+                    if (la >= 0) atomicOr(&r2[la >> 6], 1ull << (la & 63));
+                }
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+        __syncthreads();
+        if (lane == 0) n2out[a] = cnt;
+        for (int w = lane; w < rw; w += 64) {
+            A1[(size_t)a * rw + w] = r1[w];
+            A2[(size_t)a * rw + w] = r2[w];
         }
         __syncthreads();
     }
@@ -195,6 +259,7 @@ int upload_graph(const qc_graph* g, hipStream_t s) {
 }
 
 int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
+    const auto t_start = std::chrono::steady_clock::now();
     const int n = g->n;
     I.n = n;
     I.gamma = gamma;
@@ -245,6 +310,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
             }
         }
     }
+    const auto t_host = std::chrono::steady_clock::now();
     // ---- root-local ids ----
     I.orig.clear();
     std::vector<int32_t> loc(n, -1);
@@ -264,8 +330,13 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     const size_t mat_bytes = (size_t)R * I.rw * 8;
     if ((rc = I.A1.alloc(mat_bytes))) return rc;
     if ((rc = I.A2.alloc(mat_bytes))) return rc;
-    QC_HIP(hipMemsetAsync(I.A1.p, 0, mat_bytes, s));
-    QC_HIP(hipMemsetAsync(I.A2.p, 0, mat_bytes, s));
+    // LDS form: visited bitmap + both rows of one root vertex per wave
+    const size_t lds_bytes = (((size_t)n + 63) / 64 + 2 * (size_t)I.rw) * 8;
+    const bool lds_rows = !g->has_n2 && lds_bytes <= 48 * 1024;
+    if (!lds_rows) {
+        QC_HIP(hipMemsetAsync(I.A1.p, 0, mat_bytes, s));
+        QC_HIP(hipMemsetAsync(I.A2.p, 0, mat_bytes, s));
+    }
     DevCSR* G = nullptr;
     if ((rc = device_csr(g, &G, s))) return rc;
     DBuf d_loc, d_n2, d_vis;
@@ -276,14 +347,24 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     QC_HIP(hipMemcpyAsync(I.d_orig.p, I.orig.data(), (size_t)R * 4, hipMemcpyHostToDevice, s));
     int blocks = std::min(R, 2048);
     uint64_t vis_words = ((uint64_t)n + 63) / 64;
-    if (!g->has_n2) {
-        if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;
-        QC_HIP(hipMemsetAsync(d_vis.p, 0, (size_t)blocks * vis_words * 8, s));
+    if (lds_rows) {
+        int dev = 0, n_cu = 0;
+        QC_HIP(hipGetDevice(&dev));
+        QC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int lblocks = std::min(R, std::max(1, n_cu) * 16);
+        hipLaunchKernelGGL(qcdev::root_rows_lds, dim3(lblocks), dim3(64), lds_bytes, s, G->off.as<uint64_t>(),
+                           G->adj.as<uint32_t>(), d_loc.as<int32_t>(), I.d_orig.as<uint32_t>(), R, I.rw, n,
+                           I.A1.as<uint64_t>(), I.A2.as<uint64_t>(), d_n2.as<int32_t>());
+    } else {
+        if (!g->has_n2) {
+            if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;
+            QC_HIP(hipMemsetAsync(d_vis.p, 0, (size_t)blocks * vis_words * 8, s));
+        }
+        hipLaunchKernelGGL(qcdev::root_rows, dim3(blocks), dim3(64), 0, s, G->off.as<uint64_t>(),
+                           G->adj.as<uint32_t>(), G->off2.as<uint64_t>(), G->adj2.as<uint32_t>(), d_loc.as<int32_t>(),
+                           I.d_orig.as<uint32_t>(), R, I.rw, I.A1.as<uint64_t>(), I.A2.as<uint64_t>(),
+                           d_n2.as<int32_t>(), d_vis.as<unsigned long long>(), vis_words, g->has_n2 ? 1 : 0);
     }
-    hipLaunchKernelGGL(qcdev::root_rows, dim3(blocks), dim3(64), 0, s, G->off.as<uint64_t>(), G->adj.as<uint32_t>(),
-                       G->off2.as<uint64_t>(), G->adj2.as<uint32_t>(), d_loc.as<int32_t>(), I.d_orig.as<uint32_t>(), R,
-                       I.rw, I.A1.as<uint64_t>(), I.A2.as<uint64_t>(), d_n2.as<int32_t>(),
-                       d_vis.as<unsigned long long>(), vis_words, g->has_n2 ? 1 : 0);
     QC_HIP(hipGetLastError());
     if ((rc = I.d_d2r.alloc((size_t)R * 4))) return rc;
     if ((rc = I.d_d1r.alloc((size_t)R * 4))) return rc;
@@ -299,6 +380,12 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     QC_HIP(hipMemcpyAsync(I.d_mindeg.p, I.mindeg.data(), I.mindeg.size() * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(I.d_ubq.p, I.ubq.data(), I.ubq.size() * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipStreamSynchronize(s));
+    if (getenv("QCMINE_PHASES")) {
+        const auto t_dev = std::chrono::steady_clock::now();
+        fprintf(stderr, "[qcmine] root: R=%d host filter/peel %.2f ms (from call), device rows %.2f ms (%s)\n", R,
+                std::chrono::duration<double, std::milli>(t_host - t_start).count(),
+                std::chrono::duration<double, std::milli>(t_dev - t_host).count(), lds_rows ? "lds" : "global");
+    }
 
     // ---- cover vertex and covered set (main.cu:953-975) ----
     int maxd = 0, maxi = -1;

@@ -158,3 +158,21 @@ def test_trace_matches_oracle(cfg0, chunk):
     res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, trace=True)
     assert len(res.trace_subtree) == ref["stats"]["tasks"]
     assert _trace_rows(res.trace_feats, res.trace_subtree) == _trace_rows(ref["trace_feats"], sub)
+
+
+def test_capacity_growth_reruns(cfg0, monkeypatch):
+    """Capacities start small and grow by rerunning the launch: same result."""
+    g, ref = cfg0
+    monkeypatch.setenv("QCMINE_INITIAL_CAPACITY", "64")
+    _say("[gpu] capacity growth from 64")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", split_chunk=4)
+    _compare(res, ref, "growth")
+
+
+@pytest.mark.parametrize("mode", ["plain", "persistent"])
+def test_fixed_capacity_overflow_is_an_error(cfg0, mode):
+    g, _ = cfg0
+    _say(f"[gpu] fixed capacity overflow {mode}")
+    with pytest.raises(qc.QCError) as e:
+        qc.mine(g, g.gamma, g.min_size, 0, mode=mode, task_capacity=300, clique_capacity=50)
+    assert e.value.code == qc.QC_EOVERFLOW
