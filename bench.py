@@ -49,7 +49,7 @@ def algo_bytes(st: dict) -> int:
 def cpu_baseline(w, budget_s: float) -> dict:
     """Time the oracle (single thread) on a bounded sample of the workload:
     the last k root branches of its window, k doubled until the run takes
-    about budget_s / 4 seconds or more."""
+    about budget_s / 2 seconds or more."""
     import oracle
     from cuqc_amd import workloads
 
@@ -67,7 +67,7 @@ def cpu_baseline(w, budget_s: float) -> dict:
         dt = time.perf_counter() - t0
         t_spent += dt
         best = (r["stats"]["tasks"], dt, b0, b1)
-        if dt >= budget_s / 4 or b0 == w.root_begin or t_spent > budget_s:
+        if dt >= budget_s / 2 or b0 == w.root_begin or t_spent > budget_s:
             break
         k *= 4 if dt < budget_s / 64 else 2
     tasks, dt, b0, b1 = best
@@ -76,12 +76,39 @@ def cpu_baseline(w, budget_s: float) -> dict:
                       f"{w.name}: {tasks} tasks in {dt:.2f} s"}
 
 
-def run_form(qc, g, w, form: str, steps: int, warmup: int):
-    kw = dict(window=w.window)
-    if form == "plain":
-        kw.update(mode="plain")
-    else:
-        kw.update(mode="persistent")
+def cost_model_jobs(w):
+    """Training traces for the cost model: other windows and graphs than the
+    measured one (workload 0 in full, the cheap end of config 1's root
+    branches in full depth, and the first instances of the config-3 batch)."""
+    from cuqc_amd import workloads
+    jobs = []
+    g0 = workloads.graph(workloads.WORKLOADS[0])
+    jobs.append((g0, g0.gamma, g0.min_size, None))
+    g1 = workloads.graph(workloads.WORKLOADS[1])
+    jobs.append((g1, g1.gamma, g1.min_size, (3651, 4163, -1)))
+    return jobs
+
+
+def setup_forms(qc, g, w, forms):
+    """Untimed set-up: the autotuned schedule and the trained cost model."""
+    from cuqc_amd import autotune, costmodel
+    info, kws = {}, {"plain": dict(mode="plain"), "persistent": dict(mode="persistent")}
+    sched = None
+    if "tuned" in forms or "cost" in forms:
+        t0 = time.perf_counter()
+        sched = autotune.tune([g], [g.gamma], [g.min_size], [w.window], reps=1, log=_log)
+        info["autotune"] = dict(sched.kwargs(), trials=len(sched.trials), seconds=round(time.perf_counter() - t0, 2))
+        kws["tuned"] = dict(mode="persistent", **sched.kwargs())
+    if "cost" in forms:
+        t0 = time.perf_counter()
+        model, rows = costmodel.train_from_traces(cost_model_jobs(w))
+        info["cost_model"] = {"trace_rows": rows, "seconds": round(time.perf_counter() - t0, 2)}
+        kws["cost"] = dict(mode="persistent", order="cost", cost_model=model, **sched.kwargs())
+    return kws, info
+
+
+def run_form(qc, g, w, form: str, kw: dict, steps: int, warmup: int):
+    kw = dict(kw, window=w.window)
     for i in range(warmup):
         r = qc.mine(g, g.gamma, g.min_size, 0, **kw)
         _log(f"[bench] {form} warmup {i + 1}/{warmup}: enum {r.stats['t_enum_ms']:.1f} ms")
@@ -103,7 +130,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=1)
-    ap.add_argument("--forms", default="plain,persistent")
+    ap.add_argument("--forms", default="plain,persistent,tuned,cost")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -119,12 +146,13 @@ def main() -> int:
     w = workloads.WORKLOADS[a.workload]
     g = workloads.graph(w)
     forms = [f for f in a.forms.split(",") if f]
+    kws, setup = setup_forms(qc, g, w, forms)
     out = {}
     digest = None
     for f in forms:
         # the plain form is the comparison point: one untimed-warmup-free step
         st_, wu_ = (a.plain_steps, 0) if f == "plain" else (a.steps, a.warmup)
-        res, enum_ms, wall_ms, kern_ms = run_form(qc, g, w, f, st_, wu_)
+        res, enum_ms, wall_ms, kern_ms = run_form(qc, g, w, f, kws[f], st_, wu_)
         d = set_digest(res.maximal)
         if digest is not None and d != digest:
             print(f"maximal sets differ between forms ({f})", file=sys.stderr)
@@ -132,7 +160,9 @@ def main() -> int:
         digest = d
         out[f] = dict(res=res, enum_ms=sum(enum_ms) / len(enum_ms), wall_ms=sum(wall_ms) / len(wall_ms),
                       kern_ms=sum(kern_ms) / len(kern_ms))
-    best = min(forms, key=lambda f: out[f]["enum_ms"])
+    # the headline form: batched (single persistent launch), autotuned and
+    # cost-ordered when measured, else the fastest
+    best = "cost" if "cost" in out else min(forms, key=lambda f: out[f]["enum_ms"])
     st = out[best]["res"].stats
     tasks = st["tasks"]
     enum_ms = out[best]["enum_ms"]
@@ -166,8 +196,13 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                      "algorithmic_bytes": A, "kernel_ms": round(out[best]["kern_ms"], 3)},
     }
-    if "plain" in out and best != "plain":
-        line["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / enum_ms, 3)
+    if "plain" in out:
+        for f in out:
+            if f != "plain":
+                line["forms"][f]["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / out[f]["enum_ms"], 3)
+        if best != "plain":
+            line["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / enum_ms, 3)
+    line["setup"] = setup
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget_s)
     print(json.dumps(line), flush=True)

@@ -7,8 +7,11 @@ timing candidate schedules on the workload itself (or on a sample window of
 it):
 
 * ``lds_tile``    tasks up to this many vertices run from LDS (64/128/256);
-* ``split_chunk`` root-range work items of this many branches for tasks above
-                  the LDS tile (-1: never split);
+* ``split_chunk`` branch-range work items of this many branches (-1: never
+                  split);
+* ``split_min``   split only tasks above this many vertices (0: the LDS tile);
+* ``mid_tile``    tasks above the LDS tile and up to this size build task-local
+                  bit matrices in HBM (0: none, read the root matrices);
 * ``workgroups``  resident waves of the persistent launch (0: as many as fit).
 
 The search is coordinate descent from the library defaults, one parameter at
@@ -27,10 +30,13 @@ from dataclasses import asdict, dataclass, field
 
 SPACE: dict[str, list[int]] = {
     "split_chunk": [4, 16, 64, -1],
+    "split_min": [0, 512, 1024],
+    "mid_tile": [0, 512, 1024],
     "lds_tile": [64, 128, 256],
     "workgroups": [512, 1024, 2048, 0],
 }
-DEFAULT = {"lds_tile": 128, "split_chunk": 16, "workgroups": 0}
+DEFAULT = {"lds_tile": 128, "split_chunk": 16, "split_min": 0, "mid_tile": 0, "workgroups": 0}
+ORDER = ("split_chunk", "split_min", "mid_tile", "lds_tile", "workgroups")
 
 _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
 
@@ -39,12 +45,14 @@ _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
 class Schedule:
     lds_tile: int = 128
     split_chunk: int = 16
+    split_min: int = 0
+    mid_tile: int = 0
     workgroups: int = 0
     kernel_ms: float = 0.0
     trials: list = field(default_factory=list)
 
     def kwargs(self) -> dict:
-        return {"lds_tile": self.lds_tile, "split_chunk": self.split_chunk, "workgroups": self.workgroups}
+        return {k: getattr(self, k) for k in DEFAULT}
 
 
 def _digest(results) -> str:
@@ -92,7 +100,7 @@ def tune(graphs, gammas, min_sizes, windows=None, *, reps: int = 2, order: str =
 
     def measure(cfg: dict) -> float:
         nonlocal ref_stats, ref_digest
-        k = (cfg["lds_tile"], cfg["split_chunk"], cfg["workgroups"])
+        k = tuple(cfg[n] for n in DEFAULT)
         if k in seen:
             return seen[k]
         times = []
@@ -113,7 +121,7 @@ def tune(graphs, gammas, min_sizes, windows=None, *, reps: int = 2, order: str =
 
     best = dict(DEFAULT)
     best_t = measure(best)
-    for name in ("split_chunk", "lds_tile", "workgroups"):
+    for name in ORDER:
         for v in SPACE[name]:
             cand = dict(best, **{name: v})
             t = measure(cand)

@@ -70,3 +70,35 @@ def subtree_sizes(parent: np.ndarray) -> np.ndarray:
         if parent[i] >= 0:
             sub[parent[i]] += sub[i]
     return sub
+
+
+def trace_rows(results) -> tuple[np.ndarray, np.ndarray]:
+    """Features and log subtree sizes of every traced task in ``results``
+    (results of ``mine(..., trace=True)`` / ``mine_batch(..., trace=True)``)."""
+    feats, target = [], []
+    for r in results:
+        if r.trace_feats is None or len(r.trace_feats) == 0:
+            continue
+        feats.append(np.asarray(r.trace_feats, np.float32))
+        target.append(np.log(np.asarray(r.trace_subtree, np.float64)).astype(np.float32))
+    if not feats:
+        return np.zeros((0, NF), np.float32), np.zeros(0, np.float32)
+    return np.concatenate(feats), np.concatenate(target)
+
+
+def train_from_traces(jobs, steps: int = 400, lr: float = 3e-3, seed: int = 0, max_rows: int = 200_000):
+    """Log traces of the given mining jobs with the FIFO persistent kernel and
+    train the cost model on them with the GPU kernels.
+
+    jobs: iterable of (graph, gamma, min_size, window) tuples.
+    Returns (model, rows used)."""
+    from . import CostModel, mine
+
+    res = [mine(g, ga, ms, 0, mode="persistent", window=w, trace=True) for g, ga, ms, w in jobs]
+    X, y = trace_rows(res)
+    if len(y) == 0:
+        raise ValueError("the traced jobs expanded no tasks")
+    if len(y) > max_rows:
+        idx = np.random.default_rng(seed).choice(len(y), max_rows, replace=False)
+        X, y = X[idx], y[idx]
+    return CostModel.train(X, y, steps=steps, lr=lr, seed=seed), len(y)

@@ -104,8 +104,8 @@ int ensure(DBuf& b, size_t bytes) {
     return b.alloc(bytes);
 }
 
-// tasks up to this many vertices use task-local matrices in the slab
-constexpr int MID_MAX = 1024;
+// largest task that may use task-local matrices in the slab
+constexpr int MID_MAX = 4096;
 }  // namespace
 
 int enumerate_persistent(const Instance* const* insts, const qc_params* params, int count, EnumOut* outs,
@@ -171,7 +171,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 8));
     if (p.workgroups > 0) grid = std::min<uint32_t>(grid, (uint32_t)p.workgroups);
 
-    const uint32_t mid = (uint32_t)std::min(maxR, MID_MAX);
+    const int mid_tile = std::min(p.mid_tile > 0 ? (int)p.mid_tile : 0, MID_MAX);
+    const uint32_t mid = (uint32_t)std::min(maxR, mid_tile);
     const uint32_t midw = (mid + 63) / 64;
     const bool use_mid = (int)mid > TL;
     const size_t slab_b = std::max(ws_bytes_host((uint32_t)maxR, false, (uint32_t)maxrw),
@@ -283,7 +284,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.error = ws.err.as<int>();
         A.queue_mode = 1;
         A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 16 : p.split_chunk);
-        A.split_min_t = (uint32_t)TL;
+        A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : TL);
         A.q_refs = ws.q.as<TaskRef>();
         A.q_ready = ws.ready.as<unsigned int>();
         unsigned long long* ctr = ws.ctr.as<unsigned long long>();

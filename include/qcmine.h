@@ -70,6 +70,11 @@ typedef struct qc_params {
                                 require identical counters from every replay; <= 1: once */
     int32_t trace;           /* persistent mode: record the cost-model features of every expanded
                                 task and its subtree size (qc_result_trace) */
+    int32_t split_min;       /* persistent mode: split only tasks above this many vertices; 0 = the
+                                LDS tile */
+    int32_t mid_tile;        /* persistent mode: tasks above the LDS tile and up to this size
+                                (<= 4096) build task-local bit matrices in HBM; <= 0 = none (read
+                                the root matrices) */
 } qc_params;
 
 typedef struct qc_stats {
