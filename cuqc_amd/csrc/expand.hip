@@ -504,20 +504,32 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
             for (int k = c.lane; k < nl; k += WAVE) sw += c.g->d1r[w.cid[L[k]]];
             wadj_add(c, wave_sum64(sw));
         }
-        if (nrem < nrmv) {
-            if (map_empty) {
-                for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = 0;
+        if (nrem < nrmv && map_empty) {
+            for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = 0;
+        } else if (!map_empty) {
+            // exdeg = |N(i) n remaining| (recompute) or -= |N(i) n removed|
+            const bool rc = nrem < nrmv;
+            const int nl = rc ? nrem : nrmv;
+            const int* L = rc ? w.rem : w.rmv;
+            if (nl < c.W) {
+                // few vertices: test their bits (nl loads per row, not W)
+                for (int k = c.lane; k < nl; k += WAVE) w.adjc[k] = w.cpp[L[k]];
+                wsync();
+                for (int i = c.lane; i < tc; i += WAVE) {
+                    const int row = w.cpp[i];
+                    int cnt = 0;
+                    for (int k = 0; k < nl; k++) cnt += (int)gbit(c.M1, c.W, row, w.adjc[k]);
+                    w.cex[i] = rc ? cnt : w.cex[i] - cnt;
+                }
             } else {
                 clear_mask(c, w.mk0);
-                for (int k = c.lane; k < nrem; k += WAVE) set_mask_bit(w.mk0, w.cpp[w.rem[k]]);
+                for (int k = c.lane; k < nl; k += WAVE) set_mask_bit(w.mk0, w.cpp[L[k]]);
                 wsync();
-                for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = rowpop(c, c.M1, w.cpp[i], w.mk0);
+                for (int i = c.lane; i < tc; i += WAVE) {
+                    const int cnt = rowpop(c, c.M1, w.cpp[i], w.mk0);
+                    w.cex[i] = rc ? cnt : w.cex[i] - cnt;
+                }
             }
-        } else if (!map_empty) {
-            clear_mask(c, w.mk0);
-            for (int k = c.lane; k < nrmv; k += WAVE) set_mask_bit(w.mk0, w.cpp[w.rmv[k]]);
-            wsync();
-            for (int i = c.lane; i < tc; i += WAVE) w.cex[i] -= rowpop(c, c.M1, w.cpp[i], w.mk0);
         }
         wsync();
         // indegrees of remaining candidates valid under the previous bounds
@@ -596,24 +608,41 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
     WS& w = c.w;
     int thr = md(c, m + lb);
     // mask of critical members
-    clear_mask(c, w.mk0);
-    bool any = false;
+    // critical members: matrix indices listed in rmv[0..ncm)
+    int ncm = 0;
     long long sw = 0;
-    for (int k = c.lane; k < m; k += WAVE) {
-        if (w.cin[k] + w.cex[k] == thr && w.cex[k] > 0) {
-            set_mask_bit(w.mk0, w.cpp[k]);
-            any = true;
+    for (int base = 0; base < m; base += WAVE) {
+        const int k = base + c.lane;
+        const bool cr = k < m && w.cin[k] + w.cex[k] == thr && w.cex[k] > 0;
+        const uint64_t bc = ballot(cr);
+        if (cr) {
+            w.rmv[ncm + lanes_below(bc)] = w.cpp[k];
             sw += c.g->d1r[w.cid[k]];
         }
+        ncm += __popcll(bc);
     }
     wadj_add(c, wave_sum64(sw));
     wsync();
     // critical-adjacent candidates
     int ncrit = 0;
-    if (ballot(any)) {
+    if (ncm > 0) {
+        const bool sparse = ncm < c.W;
+        if (!sparse) {
+            clear_mask(c, w.mk0);
+            for (int k = c.lane; k < ncm; k += WAVE) set_mask_bit(w.mk0, w.rmv[k]);
+            wsync();
+        }
         for (int base = m; base < tc; base += WAVE) {
             int i = base + c.lane;
-            bool f = i < tc && rowpop(c, c.M1, w.cpp[i], w.mk0) > 0;
+            bool f = false;
+            if (i < tc) {
+                const int row = w.cpp[i];
+                if (sparse) {
+                    for (int k = 0; k < ncm && !f; k++) f = gbit(c.M1, c.W, row, w.rmv[k]);
+                } else {
+                    f = rowpop(c, c.M1, row, w.mk0) > 0;
+                }
+            }
             ncrit += __popcll(ballot(f));
             if (f) w.clab[i] = 4;
         }
@@ -661,20 +690,31 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
         permute(w.cl2);
     }
     // add the critical-adjacent vertices: degree updates and 2-hop counts
-    clear_mask(c, w.mk1);
+    const bool sparse = ncrit < c.W;
+    if (!sparse) clear_mask(c, w.mk1);
     sw = 0;
     for (int k = c.lane; k < ncrit; k += WAVE) {
-        set_mask_bit(w.mk1, w.cpp[m + k]);
+        if (!sparse) set_mask_bit(w.mk1, w.cpp[m + k]);
         sw += c.g->d1r[w.cid[m + k]] + c.g->d2r[w.cid[m + k]];
     }
     wadj_add(c, wave_sum64(sw));
     wsync();
     for (int i = c.lane; i < tc; i += WAVE) {
         int pp = w.cpp[i];
-        int d1 = rowpop(c, c.M1, pp, w.mk1);
+        int d1 = 0, d2 = 0;
+        if (sparse) {
+            for (int k = 0; k < ncrit; k++) {
+                const int col = w.cpp[m + k];
+                d1 += (int)gbit(c.M1, c.W, pp, col);
+                d2 += (int)gbit(c.M2, c.W, pp, col);
+            }
+        } else {
+            d1 = rowpop(c, c.M1, pp, w.mk1);
+            d2 = rowpop(c, c.M2, pp, w.mk1);
+        }
         w.cin[i] += d1;
         w.cex[i] -= d1;
-        w.adjc[i] = rowpop(c, c.M2, pp, w.mk1);
+        w.adjc[i] = d2;
     }
     wsync();
     bool fail = false;
