@@ -134,7 +134,17 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        x = torch.ones(a.copy_probe, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.copy_probe}), flush=True)
+        return 0
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the miner runs on a single device and has no multi-GPU path "
               f"(--gpus {a.gpus} requested)", file=sys.stderr)

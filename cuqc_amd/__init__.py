@@ -47,6 +47,7 @@ class _Params(C.Structure):
         ("max_depth", C.c_int32), ("task_capacity", C.c_uint64), ("clique_capacity", C.c_uint64),
         ("cost_model", C.c_void_p), ("root_begin", C.c_int64), ("root_end", C.c_int64),
         ("replays", C.c_int32), ("trace", C.c_int32), ("split_min", C.c_int32), ("mid_tile", C.c_int32),
+        ("lds_wide", C.c_int32),
     ]
 
 
@@ -284,7 +285,7 @@ def _result_from(h: C.c_void_p, rc: int) -> Result:
 
 def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
             cost_model=None, split_chunk=0, window=None, replays=1, trace=False, split_min=0,
-            mid_tile=0) -> _Params:
+            mid_tile=0, lds_wide=0) -> _Params:
     p = _Params()
     p.gamma = float(gamma)
     p.min_size = int(min_size)
@@ -303,13 +304,14 @@ def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task
     p.trace = int(bool(trace))
     p.split_min = int(split_min)
     p.mid_tile = int(mid_tile)
+    p.lds_wide = int(lds_wide)
     return p
 
 
 def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode: str = "plain",
          order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
          clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, window=None, replays: int = 1,
-         trace: bool = False, split_min: int = 0, mid_tile: int = 0) -> Result:
+         trace: bool = False, split_min: int = 0, mid_tile: int = 0, lds_wide: int = 0) -> Result:
     """Mine maximal gamma-quasi-cliques of size >= min_size (search() + RemoveNonMax()).
 
     ``window=(root_begin, root_end, max_depth)`` restricts the search to root
@@ -318,7 +320,7 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
     than raised.
     """
     p = _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task_capacity, clique_capacity,
-                cost_model, split_chunk, window, replays, trace, split_min, mid_tile)
+                cost_model, split_chunk, window, replays, trace, split_min, mid_tile, lds_wide)
     h = C.c_void_p()
     rc = _check(load_library().qc_mine(graph._h, C.byref(p), C.byref(h)), ok=(QC_OK, QC_EEMPTY))
     return _result_from(h, rc)
@@ -327,14 +329,16 @@ def mine(graph: Graph, gamma: float, min_size: int, scheduling: int = 0, *, mode
 def mine_batch(graphs: Sequence[Graph], gammas: Sequence[float], min_sizes: Sequence[int], *, scheduling: int = 0,
                order: str = "fifo", lds_tile: int = 0, workgroups: int = 0, task_capacity: int = 0,
                clique_capacity: int = 0, cost_model=None, split_chunk: int = 0, windows=None,
-               replays: int = 1, trace: bool = False, split_min: int = 0, mid_tile: int = 0) -> list[Result]:
+               replays: int = 1, trace: bool = False, split_min: int = 0, mid_tile: int = 0,
+               lds_wide: int = 0) -> list[Result]:
     """Many instances through one persistent launch."""
     n = len(graphs)
     ps = (_Params * n)()
     for i in range(n):
         ps[i] = _params(gammas[i], min_sizes[i], scheduling, "persistent", order, lds_tile, workgroups,
                         task_capacity, clique_capacity, cost_model, split_chunk,
-                        windows[i] if windows is not None else None, replays, trace, split_min, mid_tile)
+                        windows[i] if windows is not None else None, replays, trace, split_min, mid_tile,
+                        lds_wide)
     hs = (C.c_void_p * n)(*[g._h for g in graphs])
     out = (C.c_void_p * n)()
     _check(load_library().qc_mine_batch(hs, ps, n, out))

@@ -12,6 +12,8 @@ it):
 * ``split_min``   split only tasks above this many vertices (0: the LDS tile);
 * ``mid_tile``    tasks above the LDS tile and up to this size build task-local
                   bit matrices in HBM (0: none, read the root matrices);
+* ``lds_wide``    tasks above the LDS tile and up to this size keep their
+                  per-vertex arrays in LDS (-1: none, 0: 512);
 * ``workgroups``  resident waves of the persistent launch (0: as many as fit).
 
 The search is coordinate descent from the library defaults, one parameter at
@@ -29,14 +31,15 @@ import time
 from dataclasses import asdict, dataclass, field
 
 SPACE: dict[str, list[int]] = {
-    "split_chunk": [4, 16, 64, -1],
+    "split_chunk": [8, 16, 32, -1],
     "split_min": [0, 512, 1024],
     "mid_tile": [0, 512, 1024],
+    "lds_wide": [-1, 256, 512, 1024],
     "lds_tile": [64, 128, 256],
-    "workgroups": [512, 1024, 2048, 0],
+    "workgroups": [1024, 2048, 3072, 0],
 }
-DEFAULT = {"lds_tile": 128, "split_chunk": 16, "split_min": 0, "mid_tile": 0, "workgroups": 0}
-ORDER = ("split_chunk", "split_min", "mid_tile", "lds_tile", "workgroups")
+DEFAULT = {"lds_tile": 128, "split_chunk": 16, "split_min": 0, "mid_tile": 0, "lds_wide": 0, "workgroups": 0}
+ORDER = ("lds_wide", "split_chunk", "split_min", "mid_tile", "lds_tile", "workgroups")
 
 _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
 
@@ -47,6 +50,7 @@ class Schedule:
     split_chunk: int = 16
     split_min: int = 0
     mid_tile: int = 0
+    lds_wide: int = 0
     workgroups: int = 0
     kernel_ms: float = 0.0
     trials: list = field(default_factory=list)

@@ -98,25 +98,6 @@ struct WS {
     int T2;                                      // capacity rounded to 2^k
 };
 
-__host__ __device__ constexpr uint32_t pow2_at_least(uint32_t x) {
-    uint32_t p = 1;
-    while (p < x) p <<= 1;
-    return p;
-}
-
-// bytes of a workspace for `tcap` vertices; `mat` = build local matrices,
-// `mwords` = words per mask (ceil(tcap/64) in LDS mode, rw in root space)
-__host__ __device__ constexpr size_t ws_bytes(uint32_t tcap, bool mat, uint32_t mwords) {
-    uint32_t T2 = pow2_at_least(tcap < 64 ? 64 : tcap);
-    uint32_t W = (tcap + 63) / 64;
-    size_t b = 0;
-    b += (size_t)11 * T2 * 4;        // parent + child arrays
-    b += (size_t)4 * T2 * 4;         // lists / keys
-    b = (b + 15) & ~(size_t)15;
-    if (mat) b += (size_t)2 * tcap * W * 8;
-    b += (size_t)2 * mwords * 8;
-    return b;
-}
 
 __device__ inline WS make_ws(char* base, uint32_t tcap, bool mat, uint32_t mwords) {
     WS w;
@@ -155,6 +136,15 @@ __device__ inline WS make_ws(char* base, uint32_t tcap, bool mat, uint32_t mword
     return w;
 }
 
+// per-vertex arrays at `arrays` (LDS), masks of `mwords` words at `masks`
+// (root space, in the wave's HBM slab); no local matrices
+__device__ inline WS make_ws_split(char* arrays, uint32_t tcap, uint64_t* masks, uint32_t mwords) {
+    WS w = make_ws(arrays, tcap, false, 0);
+    w.mk0 = masks;
+    w.mk1 = masks + mwords;
+    return w;
+}
+
 // ---------------------------------------------------------------------------
 // per-task state
 // ---------------------------------------------------------------------------
@@ -170,12 +160,12 @@ struct Ctx {
     const uint64_t* M2;
     int W;                // words per row of M1/M2 = words per mask
     int lane;
-    LStats st;
+    LStats* st;           // counters of this wave (LDS; lane 0 updates them)
     uint32_t depth;
     uint32_t cap;         // largest task the workspace holds
     uint32_t inst;        // instance of the task being expanded
     long long slot;       // queue slot of the task being expanded (trace)
-    unsigned long long ph[PH_COUNT];  // phase cycles (diagnostics)
+    unsigned long long* ph;           // phase cycles (LDS, diagnostics)
     long long ph_t;                   // start of the running phase
 };
 
@@ -183,12 +173,13 @@ struct Ctx {
 __device__ __forceinline__ void ph_mark(Ctx& c, int k) {
     if (c.a->phase) {
         long long n = clock64();
-        c.ph[k] += (unsigned long long)(n - c.ph_t);
+        if (c.lane == 0) c.ph[k] += (unsigned long long)(n - c.ph_t);
         c.ph_t = n;
     }
 }
 __device__ __forceinline__ void ph_start(Ctx& c) {
-    for (int k = 0; k < PH_COUNT; k++) c.ph[k] = 0;
+    if (c.lane == 0)
+        for (int k = 0; k < PH_COUNT; k++) c.ph[k] = 0;
     c.ph_t = clock64();
 }
 __device__ __forceinline__ void ph_flush(Ctx& c) {
@@ -238,7 +229,7 @@ __device__ __forceinline__ int rowpop(const Ctx& c, const uint64_t* M, int row, 
 // W_adj bookkeeping: adjacency entries the reference's list scans visit
 // (restricted to root vertices), summed over ids
 __device__ __forceinline__ void wadj_add(Ctx& c, long long x) {
-    if (c.lane == 0) c.st.wadj += (unsigned long long)x;
+    if (c.lane == 0) c.st->wadj += (unsigned long long)x;
 }
 
 __device__ inline void clear_mask(Ctx& c, uint64_t* mk) {
@@ -474,8 +465,8 @@ __device__ inline bool members_fail(Ctx& c, int m, int ub, int lb, int med) {
 // ---------------------------------------------------------------------------
 // map_empty: the reference's id->position map holds no entry during this call
 // (critical-vertex pruning without critical-adjacent candidates).
-__device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb, int& med, int nind,
-                               bool map_empty) {
+__device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb,
+                                                              int& med, int nind, bool map_empty) {
     WS& w = c.w;
     int V = indeg_dist(c, nind);
     if (V < 0) sort_desc_int(c, w.cind, nind);
@@ -504,8 +495,12 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
             for (int k = c.lane; k < nl; k += WAVE) sw += c.g->d1r[w.cid[L[k]]];
             wadj_add(c, wave_sum64(sw));
         }
+        // only members and remaining candidates are updated: the removed
+        // ones never return, so their exdeg is never read again
+        const int nupd = m + nrem;
+#define UPD_IDX(j) ((j) < m ? (j) : w.rem[(j) - m])
         if (nrem < nrmv && map_empty) {
-            for (int i = c.lane; i < tc; i += WAVE) w.cex[i] = 0;
+            for (int j = c.lane; j < nupd; j += WAVE) w.cex[UPD_IDX(j)] = 0;
         } else if (!map_empty) {
             // exdeg = |N(i) n remaining| (recompute) or -= |N(i) n removed|
             const bool rc = nrem < nrmv;
@@ -515,7 +510,8 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
                 // few vertices: test their bits (nl loads per row, not W)
                 for (int k = c.lane; k < nl; k += WAVE) w.adjc[k] = w.cpp[L[k]];
                 wsync();
-                for (int i = c.lane; i < tc; i += WAVE) {
+                for (int j = c.lane; j < nupd; j += WAVE) {
+                    const int i = UPD_IDX(j);
                     const int row = w.cpp[i];
                     int cnt = 0;
                     for (int k = 0; k < nl; k++) cnt += (int)gbit(c.M1, c.W, row, w.adjc[k]);
@@ -525,12 +521,14 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
                 clear_mask(c, w.mk0);
                 for (int k = c.lane; k < nl; k += WAVE) set_mask_bit(w.mk0, w.cpp[L[k]]);
                 wsync();
-                for (int i = c.lane; i < tc; i += WAVE) {
+                for (int j = c.lane; j < nupd; j += WAVE) {
+                    const int i = UPD_IDX(j);
                     const int cnt = rowpop(c, c.M1, w.cpp[i], w.mk0);
                     w.cex[i] = rc ? cnt : w.cex[i] - cnt;
                 }
             }
         }
+#undef UPD_IDX
         wsync();
         // indegrees of remaining candidates valid under the previous bounds
         int nv = 0;
@@ -601,10 +599,14 @@ __device__ bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb
 }
 
 // ---------------------------------------------------------------------------
-// critical vertex pruning, main.cu:1526-1707; returns 0, 1 or 2 like the
-// reference
+// critical vertex pruning, main.cu:1526-1707, up to its closing degree
+// pruning: returns 2 when the task fails (the reference's 2), else 0 with the
+// arguments of the degree-pruning call that ends the step (candidate
+// indegrees in cind[0..nind), map_empty).  The caller runs that call; its
+// result gives the reference's 0 / 1.
 // ---------------------------------------------------------------------------
-__device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& lb, int& med) {
+__device__ __attribute__((always_inline)) int critical_prep(Ctx& c, int tc, int& nc, int& m, int lb, int& nind_out,
+                                                            bool& map_empty) {
     WS& w = c.w;
     int thr = md(c, m + lb);
     // mask of critical members
@@ -659,7 +661,9 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
             nind += __popcll(bo);
         }
         wsync();
-        return degree_pruning(c, tc, nc, m, ub, lb, med, nind, true) ? 1 : 0;
+        nind_out = nind;
+        map_empty = true;
+        return 0;
     }
     // stable partition of [m, tc): label 4 first (perm in rmv, temp in cind)
     {
@@ -738,7 +742,9 @@ __device__ int critical_pruning(Ctx& c, int& tc, int& nc, int& m, int& ub, int& 
         nind += __popcll(bo);
     }
     wsync();
-    return degree_pruning(c, tc, nc, m, ub, lb, med, nind, false) ? 1 : 0;
+    nind_out = nind;
+    map_empty = false;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -765,8 +771,10 @@ __device__ void emit_clique(Ctx& c, const int* ids, int len) {
         r.inst = c.inst;
         a.clq_refs[ri] = r;
     }
-    c.st.raw++;
-    c.st.sum_clq += len;
+    if (c.lane == 0) {
+        c.st->raw++;
+        c.st->sum_clq += len;
+    }
 }
 
 // clique test, main.cu:1974-1995
@@ -860,14 +868,16 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
     return uni(b);
 }
 
-__device__ void write_child(Ctx& c, int tc, int mem, int gap) {
+__device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int mem, int gap) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
     if (c.g->max_depth >= 0 && (int)c.depth + 1 > c.g->max_depth) {
         // outside the depth window: counted, never expanded
-        c.st.children++;
-        c.st.sum_tw += tc;
-        c.st.beyond++;
+        if (c.lane == 0) {
+            c.st->children++;
+            c.st->sum_tw += tc;
+            c.st->beyond++;
+        }
         return;
     }
     for (int i = c.lane; i < tc; i += WAVE) q_key(w.clab[i], w.cin[i], w.cex[i], w.cl2[i], w.cid[i], w.khi[i], w.klo[i]);
@@ -944,8 +954,10 @@ __device__ void write_child(Ctx& c, int tc, int mem, int gap) {
         __threadfence();
         if (c.lane == 0) __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    c.st.children++;
-    c.st.sum_tw += tc;
+    if (c.lane == 0) {
+        c.st->children++;
+        c.st->sum_tw += tc;
+    }
 }
 
 // first index in [from, t) where pred holds, else t
@@ -963,16 +975,18 @@ __device__ __forceinline__ int first_where(Ctx& c, int from, int t, P pred) {
 // one task record, main.cu:1060-1181
 // ---------------------------------------------------------------------------
 template <bool LOCAL>
-__device__ void process_task(Ctx& c, const TaskRef ref) {
+__device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRef ref) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
     const int t0 = (int)ref.t;
     c.depth = ref.depth;
     const bool item = ref.b1 > 0;
     if (!item) {  // a split item's task is counted by its prepare step
-        c.st.tasks++;
-        c.st.sum_t += t0;
-        if ((uint32_t)t0 > c.a->small_max) c.st.large++;
+        if (c.lane == 0) {
+            c.st->tasks++;
+            c.st->sum_t += t0;
+            if ((uint32_t)t0 > c.a->small_max) c.st->large++;
+        }
     }
 
     for (int i = c.lane; i < t0; i += WAVE) {
@@ -1250,55 +1264,103 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
             if (stop) break;
         }
         if (!item && b < bbeg) continue;
-        // child record: members, branch vertex, remaining in order
-        int tc = tp, m = nm, cc = nc;
-        for (int i = c.lane; i < tc; i += WAVE) {
-            int src = i < m ? i : (i == m ? tc - 1 : i - 1);
-            w.cid[i] = w.pid[src];
-            w.cpp[i] = PM(src);
-            int lab = w.plab[src];
-            if (cov > 0 && i >= nm + 1 && i <= nm + cov) lab = 0;
-            if (i == m) lab = 1;
-            w.clab[i] = lab;
-            w.cin[i] = w.pin[src];
-            w.cex[i] = w.pex[src];
-            w.cl2[i] = w.pl2[src];
+        // child record (main.cu:1094-1180 with add_one_vertex, main.cu:1465-1523,
+        // and diameter pruning, main.cu:1709-1732, applied while it is built):
+        // members, then the branch vertex (parent position tp - 1) as a
+        // member, then the candidates inside N2(v) in parent order with their
+        // degrees updated for v; candidates outside N2(v) follow with id and
+        // matrix index only (label -1: degree pruning counts them as removed)
+        if (c.lane == 0) c.st->branches++;
+        const int vsrc = tp - 1;
+        const int vpp = PM(vsrc);
+        wadj_add(c, (long long)c.g->d1r[w.pid[vsrc]] + c.g->d2r[w.pid[vsrc]]);
+        for (int i = c.lane; i < nm; i += WAVE) {
+            const int pp = PM(i);
+            const int a1 = (int)gbit(c.M1, c.W, vpp, pp);
+            w.cid[i] = w.pid[i];
+            w.cpp[i] = pp;
+            w.clab[i] = w.plab[i];
+            w.cin[i] = w.pin[i] + a1;
+            w.cex[i] = w.pex[i] - a1;
+            w.cl2[i] = w.pl2[i];
+        }
+        if (c.lane == 0) {
+            w.cid[nm] = w.pid[vsrc];
+            w.cpp[nm] = vpp;
+            w.clab[nm] = 1;
+            w.cin[nm] = w.pin[vsrc];
+            w.cex[nm] = w.pex[vsrc];
+            w.cl2[nm] = w.pl2[vsrc];
+        }
+        int m = nm + 1, cc = nc - 1;
+        // pass 1: 2-hop survivors per 64-candidate chunk (masks in adjc)
+        uint64_t* smask = (uint64_t*)w.adjc;
+        int ns = 0;
+        for (int base = nm, k = 0; base < vsrc; base += WAVE, k++) {
+            const int q = base + c.lane;
+            const bool ok = q < vsrc && gbit(c.M2, c.W, vpp, PM(q));
+            const uint64_t bo = ballot(ok);
+            if (c.lane == 0) smask[k] = bo;
+            ns += __popcll(bo);
         }
         wsync();
-        // add one vertex, main.cu:1465-1523
-        c.st.branches++;
-        const int vpp = PM(tc - 1);
-        wadj_add(c, (long long)c.g->d1r[w.pid[tc - 1]] + c.g->d2r[w.pid[tc - 1]]);
-        m++;
-        cc--;
-        for (int i = c.lane; i < tc; i += WAVE) {
-            if (gbit(c.M1, c.W, vpp, w.cpp[i])) {
-                w.cin[i]++;
-                w.cex[i]--;
+        // pass 2: place survivors (with indegrees in cind) and the others
+        int nsd = 0, ndd = 0;
+        for (int base = nm, k = 0; base < vsrc; base += WAVE, k++) {
+            const int q = base + c.lane;
+            const bool in = q < vsrc;
+            const uint64_t bo = smask[k];
+            const bool ok = in && ((bo >> c.lane) & 1ull);
+            const uint64_t bx = ballot(in && !ok);
+            if (ok) {
+                const int r = lanes_below(bo);
+                const int d = m + nsd + r;
+                const int pp = PM(q);
+                const int a1 = (int)gbit(c.M1, c.W, vpp, pp);
+                const int cin_ = w.pin[q] + a1;
+                w.cid[d] = w.pid[q];
+                w.cpp[d] = pp;
+                w.clab[d] = 0;
+                w.cin[d] = cin_;
+                w.cex[d] = w.pex[q] - a1;
+                w.cl2[d] = w.pl2[q];
+                w.cind[nsd + r] = cin_;
+            } else if (in) {
+                const int d = m + ns + ndd + lanes_below(bx);
+                w.cid[d] = w.pid[q];
+                w.cpp[d] = PM(q);
+                w.clab[d] = -1;
             }
+            nsd += __popcll(bo);
+            ndd += __popcll(bx);
         }
-        wsync();
-        // diameter pruning, main.cu:1709-1732
-        int nind = 0;
-        for (int base = m; base < tc; base += WAVE) {
-            int i = base + c.lane;
-            bool in = i < tc;
-            bool ok = in && gbit(c.M2, c.W, vpp, w.cpp[i]);
-            uint64_t bo = ballot(ok);
-            if (ok) w.cind[nind + lanes_below(bo)] = w.cin[i];
-            if (in) w.clab[i] = ok ? 0 : -1;
-            nind += __popcll(bo);
-        }
+        int tc = m + ns + ndd;
+        int nind = ns;
         wsync();
         ph_mark(c, PH_ADD);
         int ub = 0, lb = 0, med = 0;
-        if (degree_pruning(c, tc, cc, m, ub, lb, med, nind, false)) {
-            if (m >= c.g->min_size) check_for_clique(c, m);
-            ph_mark(c, PH_PRUNE);
-            continue;
+        // degree pruning, then critical-vertex pruning, which ends in a
+        // second degree pruning (one call site: the call is inlined)
+        bool first = true, map_empty = false, skip = false;
+        int r = 0;
+        for (;;) {
+            const bool fail = degree_pruning(c, tc, cc, m, ub, lb, med, nind, map_empty);
+            if (!first) {
+                r = fail ? 1 : 0;
+                break;
+            }
+            if (fail) {
+                if (m >= c.g->min_size) check_for_clique(c, m);
+                skip = true;
+                break;
+            }
+            if (critical_prep(c, tc, cc, m, lb, nind, map_empty) == 2) {
+                skip = true;
+                break;
+            }
+            first = false;
         }
-        int r = critical_pruning(c, tc, cc, m, ub, lb, med);
-        if (r == 2) {
+        if (skip) {
             ph_mark(c, PH_PRUNE);
             continue;
         }
@@ -1317,18 +1379,19 @@ __device__ void process_task(Ctx& c, const TaskRef ref) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void flush_stats(Ctx& c, DevStats* d) {
     if (c.lane == 0) {
-        if (c.st.tasks) atomicAdd(&d->tasks, c.st.tasks);
-        if (c.st.branches) atomicAdd(&d->branches, c.st.branches);
-        if (c.st.children) atomicAdd(&d->children, c.st.children);
-        if (c.st.raw) atomicAdd(&d->raw_cliques, c.st.raw);
-        if (c.st.sum_t) atomicAdd(&d->sum_t, c.st.sum_t);
-        if (c.st.sum_tw) atomicAdd(&d->sum_t_written, c.st.sum_tw);
-        if (c.st.sum_clq) atomicAdd(&d->sum_clique_len, c.st.sum_clq);
-        if (c.st.large) atomicAdd(&d->large_tasks, c.st.large);
-        if (c.st.beyond) atomicAdd(&d->beyond, c.st.beyond);
-        if (c.st.wadj) atomicAdd(&d->w_adj, c.st.wadj);
+        const LStats& t = *c.st;
+        if (t.tasks) atomicAdd(&d->tasks, t.tasks);
+        if (t.branches) atomicAdd(&d->branches, t.branches);
+        if (t.children) atomicAdd(&d->children, t.children);
+        if (t.raw) atomicAdd(&d->raw_cliques, t.raw);
+        if (t.sum_t) atomicAdd(&d->sum_t, t.sum_t);
+        if (t.sum_tw) atomicAdd(&d->sum_t_written, t.sum_tw);
+        if (t.sum_clq) atomicAdd(&d->sum_clique_len, t.sum_clq);
+        if (t.large) atomicAdd(&d->large_tasks, t.large);
+        if (t.beyond) atomicAdd(&d->beyond, t.beyond);
+        if (t.wadj) atomicAdd(&d->w_adj, t.wadj);
+        *c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     }
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 }
 __device__ __forceinline__ void flush_stats(Ctx& c) { flush_stats(c, c.a->stats); }
 
@@ -1375,7 +1438,11 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     c.lane = lane_id();
     c.w = make_ws(lds, TL, true, (TL + 63) / 64);
     c.cap = TL;
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    __shared__ LStats s_st;
+    __shared__ unsigned long long s_ph[PH_COUNT];
+    c.st = &s_st;
+    c.ph = s_ph;
+    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     task_loop<true>(c);
 }
@@ -1391,7 +1458,11 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     char* base = a.slab + (size_t)blockIdx.x * a.slab_bytes;
     c.w = make_ws(base, a.slab_tcap, false, a.slab_mwords);
     c.cap = a.slab_tcap;
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    __shared__ LStats s_st;
+    __shared__ unsigned long long s_ph[PH_COUNT];
+    c.st = &s_st;
+    c.ph = s_ph;
+    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     task_loop<false>(c);
 }
@@ -1404,20 +1475,22 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
 template <int TL>
 __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __restrict__ ap) {
     const ExpandArgs& a = *ap;
-    constexpr size_t BYTES = ws_bytes(TL, true, (TL + 63) / 64);
-    __shared__ __attribute__((aligned(16))) char lds[BYTES];
+    // dynamic LDS: max(TL workspace with local matrices, arrays of wide tasks)
+    extern __shared__ __attribute__((aligned(16))) char lds[];
     Ctx c;
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
     c.slot = -1;
     c.lane = lane_id();
-    const WS wl = make_ws(lds, TL, true, (TL + 63) / 64);
     char* const slab = a.slab + (size_t)blockIdx.x * a.slab_bytes;
-    const WS wg = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
     const uint32_t mid = a.mid_tcap;
-    const WS wm = make_ws(slab, mid, true, (mid + 63) / 64);
-    c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t wide = a.lds_wide;
+    __shared__ LStats s_st;
+    __shared__ unsigned long long s_ph[PH_COUNT];
+    c.st = &s_st;
+    c.ph = s_ph;
+    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     constexpr int WAIT_LIMIT = 1 << 22;
     int idle = 0;      // consecutive empty polls (back-off)
@@ -1484,20 +1557,31 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                 sinst = ref.inst;
             }
             c.slot = ref.b1 > 0 ? (long long)ref.bound : (long long)h;
+            // workspace: LDS with local matrices, LDS arrays with root
+            // matrices, slab with local matrices, or slab in root space
+            const bool fits = ref.t <= a.slab_tcap;
+            const bool local = ref.t <= (uint32_t)TL || (ref.t > wide && ref.t <= mid);
+            // (built per task: only one workspace descriptor is live)
             if (ref.t <= (uint32_t)TL) {
-                c.w = wl;
+                c.w = make_ws(lds, TL, true, (TL + 63) / 64);
                 c.cap = TL;
-                process_task<true>(c, ref);
+            } else if (ref.t <= wide) {
+                c.w = make_ws_split(lds, wide, (uint64_t*)(slab + ws_bytes(a.slab_tcap, false, 0)),
+                                    a.slab_mwords);
+                c.cap = wide;
             } else if (ref.t <= mid) {
-                c.w = wm;
+                c.w = make_ws(slab, mid, true, (mid + 63) / 64);
                 c.cap = mid;
-                process_task<true>(c, ref);
-            } else if (ref.t <= a.slab_tcap) {
-                c.w = wg;
+            } else {
+                c.w = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
                 c.cap = a.slab_tcap;
+            }
+            if (!fits) {
+                if (c.lane == 0) atomicOr(a.error, 2);
+            } else if (local) {
+                process_task<true>(c, ref);
+            } else {
                 process_task<false>(c, ref);
-            } else if (c.lane == 0) {
-                atomicOr(a.error, 2);
             }
             wsync();
             __threadfence();

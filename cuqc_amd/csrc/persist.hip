@@ -65,14 +65,7 @@ namespace {
 // and, in the same bytes, the local-matrix workspace for tasks between the
 // LDS tile and the mid size (a wave runs one task at a time).
 size_t ws_bytes_host(uint32_t tcap, bool mat, uint32_t mwords) {
-    uint32_t T2 = 64;
-    while (T2 < tcap) T2 <<= 1;
-    const uint32_t W = (tcap + 63) / 64;
-    size_t b = (size_t)15 * T2 * 4;
-    b = (b + 15) & ~(size_t)15;
-    if (mat) b += (size_t)2 * tcap * W * 8;
-    b += (size_t)2 * mwords * 8;
-    return (b + 255) & ~(size_t)255;
+    return (ws_bytes(tcap, mat, mwords) + 255) & ~(size_t)255;
 }
 
 struct RunState {
@@ -166,9 +159,13 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     int dev = 0, n_cu = 0, per_cu = 0;
     QC_HIP(hipGetDevice(&dev));
     QC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+    // wide tasks: per-vertex arrays in LDS up to this size (dynamic LDS)
+    const int wide_req = p.lds_wide < 0 ? 0 : (p.lds_wide > 0 ? (int)p.lds_wide : 512);
+    const uint32_t wide = wide_req > TL ? (uint32_t)std::min(std::min(wide_req, 1024), std::max(maxR, TL)) : 0u;
+    const size_t lds_b = std::max(ws_bytes(TL, true, (TL + 63) / 64), wide ? ws_bytes(wide, false, 0) : (size_t)0);
+    QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds_b));
     // every wave must be resident: the queue hand-off waits on producers
-    uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 8));
+    uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 16));
     if (p.workgroups > 0) grid = std::min<uint32_t>(grid, (uint32_t)p.workgroups);
 
     const int mid_tile = std::min(p.mid_tile > 0 ? (int)p.mid_tile : 0, MID_MAX);
@@ -280,6 +277,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.slab_tcap = (uint32_t)maxR;
         A.slab_mwords = (uint32_t)maxrw;
         A.mid_tcap = use_mid ? mid : 0;
+        A.lds_wide = wide;
         A.stats = ws.stats.as<DevStats>();
         A.error = ws.err.as<int>();
         A.queue_mode = 1;
@@ -316,7 +314,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
                            ws.ready.as<unsigned int>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
                            ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
                            (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent);
-        hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, 0, s);
+        hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, lds_b, s);
         hipError_t ce = hipStreamEndCapture(s, &graph);
         if (le != hipSuccess || ce != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);

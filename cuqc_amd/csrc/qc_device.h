@@ -14,6 +14,28 @@
 // stride (in 64-bit words) between the persistent queue's shared counters
 constexpr int QSTRIDE = 32;
 
+__host__ __device__ constexpr uint32_t pow2_at_least(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Bytes of a per-wave workspace for tasks of up to `tcap` vertices: 15
+// per-vertex int arrays of pow2(tcap) entries, then (`mat`) the task-local
+// 1-hop / 2-hop matrices (tcap rows of ceil(tcap/64) words), then two masks of
+// `mwords` words (ceil(tcap/64) with local matrices, rw in root space).
+__host__ __device__ constexpr size_t ws_bytes(uint32_t tcap, bool mat, uint32_t mwords) {
+    uint32_t T2 = pow2_at_least(tcap < 64 ? 64 : tcap);
+    uint32_t W = (tcap + 63) / 64;
+    size_t b = 0;
+    b += (size_t)11 * T2 * 4;        // parent + child arrays
+    b += (size_t)4 * T2 * 4;         // lists / keys
+    b = (b + 15) & ~(size_t)15;
+    if (mat) b += (size_t)2 * tcap * W * 8;
+    b += (size_t)2 * mwords * 8;
+    return b;
+}
+
 struct TaskRef {
     uint64_t off;
     uint32_t t;
@@ -109,7 +131,8 @@ struct ExpandArgs {
     uint32_t slab_mwords;            // mask words per slab (root-space row words)
     uint32_t mid_tcap;               // persistent mode: tasks above the LDS tile and up to
                                      // this size use task-local matrices in the slab (0: none)
-    uint32_t pad3;
+    uint32_t lds_wide;               // persistent mode: tasks above the LDS tile and up to
+                                     // this size keep their per-vertex arrays in LDS (0: none)
 
     // persistent mode: n_buckets FIFO queues of task refs (bucket b owns
     // slots [b q_cap, (b+1) q_cap) of q_refs / q_ready and the counters

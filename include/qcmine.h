@@ -75,6 +75,8 @@ typedef struct qc_params {
     int32_t mid_tile;        /* persistent mode: tasks above the LDS tile and up to this size
                                 (<= 4096) build task-local bit matrices in HBM; <= 0 = none (read
                                 the root matrices) */
+    int32_t lds_wide;        /* persistent mode: tasks above the LDS tile and up to this size
+                                (<= 1024) keep their per-vertex arrays in LDS; 0 = 512, < 0 = none */
 } qc_params;
 
 typedef struct qc_stats {

@@ -176,3 +176,22 @@ def test_fixed_capacity_overflow_is_an_error(cfg0, mode):
     with pytest.raises(qc.QCError) as e:
         qc.mine(g, g.gamma, g.min_size, 0, mode=mode, task_capacity=300, clique_capacity=50)
     assert e.value.code == qc.QC_EOVERFLOW
+
+
+@pytest.mark.parametrize("lds_wide", [-1, 128, 1024])
+def test_config0_lds_wide(cfg0, lds_wide):
+    """Tasks above the 64-vertex LDS tile with their arrays in LDS (or not)."""
+    g, ref = cfg0
+    _say(f"[gpu] config0 lds_wide={lds_wide}")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, lds_wide=lds_wide, split_chunk=3)
+    _compare(res, ref, f"lds_wide {lds_wide}")
+
+
+@pytest.mark.parametrize("name,build,gamma,ms", SMALL_CASES, ids=[c[0] for c in SMALL_CASES])
+def test_small_lds_wide(name, build, gamma, ms):
+    off, adj = build()
+    ref = oracle.mine(off, adj, gamma, ms)
+    g = qc.Graph.from_csr(off, adj)
+    _say(f"[gpu] small lds_wide {name}")
+    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, lds_wide=256, split_chunk=2)
+    _compare(res, ref, name)
