@@ -30,6 +30,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel (expand_persistent), from the
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes recorded under profiles/ for the
+# default schedule, each scaled by the counter's reading for a device copy
+# of known size (FETCH x2.0, WRITE x1.0); None where not measured.
+TRAFFIC: dict = {}
+
 
 def _log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
@@ -79,49 +85,109 @@ def cpu_baseline(w, budget_s: float) -> dict:
 def cost_model_jobs(w):
     """Training traces for the cost model: other windows and graphs than the
     measured one (workload 0 in full, the cheap end of config 1's root
-    branches in full depth, and the first instances of the config-3 batch)."""
+    branches in full depth, and the first 8 instances of the config-3 batch
+    in their windows)."""
     from cuqc_amd import workloads
     jobs = []
-    g0 = workloads.graph(workloads.WORKLOADS[0])
-    jobs.append((g0, g0.gamma, g0.min_size, None))
+    if w.config != 0:
+        g0 = workloads.graph(workloads.WORKLOADS[0])
+        jobs.append((g0, g0.gamma, g0.min_size, None))
     g1 = workloads.graph(workloads.WORKLOADS[1])
     jobs.append((g1, g1.gamma, g1.min_size, (3651, 4163, -1)))
+    w3 = workloads.WORKLOADS[3]
+    first = 8 if w.config != 3 else 0
+    for i in range(first):
+        g3 = workloads.graph(w3, i)
+        jobs.append((g3, g3.gamma, g3.min_size, w3.window))
     return jobs
 
 
-def setup_forms(qc, g, w, forms):
+class Job:
+    """The graphs of one workload and how each form mines them: one call per
+    graph for the plain form, one batched persistent launch otherwise."""
+
+    def __init__(self, qc, w):
+        from cuqc_amd import workloads
+        self.qc, self.w = qc, w
+        self.gs = workloads.graphs(w)
+        self.gammas = [g.gamma for g in self.gs]
+        self.min_sizes = [g.min_size for g in self.gs]
+        self.windows = [w.window] * len(self.gs)
+
+    def run(self, kw: dict) -> dict:
+        qc = self.qc
+        if kw.get("mode") == "plain" or len(self.gs) == 1:
+            res = [qc.mine(g, g.gamma, g.min_size, 0, window=self.w.window, **kw) for g in self.gs]
+            enum_ms = sum(r.stats["t_enum_ms"] for r in res)
+            kern_ms = sum(r.stats["t_kernel_ms"] for r in res)
+        else:
+            kb = {k: v for k, v in kw.items() if k != "mode"}
+            res = qc.mine_batch(self.gs, self.gammas, self.min_sizes, windows=self.windows, **kb)
+            enum_ms = res[0].stats["t_enum_ms"]
+            kern_ms = res[0].stats["t_kernel_ms"]
+        st = {k: sum(r.stats[k] for r in res) for k in ("tasks", "raw_cliques", "sum_t", "sum_t_written",
+                                                        "sum_clique_len", "w_adj")}
+        h = hashlib.sha256()
+        for r in res:
+            h.update(set_digest(r.maximal).encode())
+        return dict(res=res, stats=st, enum_ms=enum_ms, kern_ms=kern_ms, digest=h.hexdigest(),
+                    maximal=sum(len(r.maximal) for r in res))
+
+
+def setup_forms(job: Job, forms):
     """Untimed set-up: the autotuned schedule and the trained cost model."""
     from cuqc_amd import autotune, costmodel
     info, kws = {}, {"plain": dict(mode="plain"), "persistent": dict(mode="persistent")}
     sched = None
     if "tuned" in forms or "cost" in forms:
         t0 = time.perf_counter()
-        sched = autotune.tune([g], [g.gamma], [g.min_size], [w.window], reps=1, log=_log)
+        sched = autotune.tune(job.gs, job.gammas, job.min_sizes, job.windows, reps=1, log=_log)
         info["autotune"] = dict(sched.kwargs(), trials=len(sched.trials), seconds=round(time.perf_counter() - t0, 2))
         kws["tuned"] = dict(mode="persistent", **sched.kwargs())
     if "cost" in forms:
         t0 = time.perf_counter()
-        model, rows = costmodel.train_from_traces(cost_model_jobs(w))
+        model, rows = costmodel.train_from_traces(cost_model_jobs(job.w))
         info["cost_model"] = {"trace_rows": rows, "seconds": round(time.perf_counter() - t0, 2)}
         kws["cost"] = dict(mode="persistent", order="cost", cost_model=model, **sched.kwargs())
     return kws, info
 
 
-def run_form(qc, g, w, form: str, kw: dict, steps: int, warmup: int):
-    kw = dict(kw, window=w.window)
+def run_form(job: Job, form: str, kw: dict, steps: int, warmup: int) -> dict:
     for i in range(warmup):
-        r = qc.mine(g, g.gamma, g.min_size, 0, **kw)
-        _log(f"[bench] {form} warmup {i + 1}/{warmup}: enum {r.stats['t_enum_ms']:.1f} ms")
-    enum_ms, wall_ms, kern_ms, res = [], [], [], None
+        o = job.run(kw)
+        _log(f"[bench] {job.w.name} {form} warmup {i + 1}/{warmup}: enum {o['enum_ms']:.1f} ms")
+    runs = []
     for i in range(steps):
         t0 = time.perf_counter()
-        res = qc.mine(g, g.gamma, g.min_size, 0, **kw)
-        wall_ms.append((time.perf_counter() - t0) * 1e3)
-        enum_ms.append(res.stats["t_enum_ms"])
-        kern_ms.append(res.stats["t_kernel_ms"])
-        _log(f"[bench] {form} step {i + 1}/{steps}: enum {enum_ms[-1]:.1f} ms, kernel {kern_ms[-1]:.1f} ms, "
-             f"{res.stats['tasks']} tasks")
-    return res, enum_ms, wall_ms, kern_ms
+        o = job.run(kw)
+        o["wall_ms"] = (time.perf_counter() - t0) * 1e3
+        runs.append(o)
+        _log(f"[bench] {job.w.name} {form} step {i + 1}/{steps}: enum {o['enum_ms']:.1f} ms, "
+             f"kernel {o['kern_ms']:.1f} ms, {o['stats']['tasks']} tasks")
+    last = runs[-1]
+    return dict(last, enum_ms=sum(r["enum_ms"] for r in runs) / steps, kern_ms=sum(r["kern_ms"] for r in runs) / steps,
+                wall_ms=sum(r["wall_ms"] for r in runs) / steps)
+
+
+def measure(job: Job, forms, steps: int, warmup: int, plain_steps: int):
+    kws, setup = setup_forms(job, forms)
+    out, digest = {}, None
+    for f in forms:
+        # the plain form is the comparison point: no warmup
+        st_, wu_ = (plain_steps, 0) if f == "plain" else (steps, warmup)
+        o = run_form(job, f, kws[f], st_, wu_)
+        if digest is not None and o["digest"] != digest:
+            raise SystemExit(f"maximal sets differ between forms ({f})")
+        digest = o["digest"]
+        out[f] = o
+    table = {f: {"enum_ms": round(o["enum_ms"], 3), "wall_ms": round(o["wall_ms"], 3),
+                 "kernel_ms": round(o["kern_ms"], 3),
+                 "tasks_per_s": round(o["stats"]["tasks"] / (o["enum_ms"] / 1e3), 1)} for f, o in out.items()}
+    if "plain" in out:
+        for f in out:
+            if f != "plain":
+                table[f]["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / out[f]["enum_ms"], 3)
+    return out, table, setup, digest
 
 
 def main() -> int:
@@ -132,6 +198,9 @@ def main() -> int:
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--forms", default="plain,persistent,tuned,cost")
     ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--batch-workload", type=int, default=3,
+                    help="also measure this batch workload (one launch vs one plain run per instance); -1: skip")
+    ap.add_argument("--batch-forms", default="plain,persistent,cost")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -154,31 +223,19 @@ def main() -> int:
     from cuqc_amd import workloads
 
     w = workloads.WORKLOADS[a.workload]
-    g = workloads.graph(w)
+    job = Job(qc, w)
     forms = [f for f in a.forms.split(",") if f]
-    kws, setup = setup_forms(qc, g, w, forms)
-    out = {}
-    digest = None
-    for f in forms:
-        # the plain form is the comparison point: one untimed-warmup-free step
-        st_, wu_ = (a.plain_steps, 0) if f == "plain" else (a.steps, a.warmup)
-        res, enum_ms, wall_ms, kern_ms = run_form(qc, g, w, f, kws[f], st_, wu_)
-        d = set_digest(res.maximal)
-        if digest is not None and d != digest:
-            print(f"maximal sets differ between forms ({f})", file=sys.stderr)
-            return 1
-        digest = d
-        out[f] = dict(res=res, enum_ms=sum(enum_ms) / len(enum_ms), wall_ms=sum(wall_ms) / len(wall_ms),
-                      kern_ms=sum(kern_ms) / len(kern_ms))
+    out, table, setup, digest = measure(job, forms, a.steps, a.warmup, a.plain_steps)
     # the headline form: batched (single persistent launch), autotuned and
     # cost-ordered when measured, else the fastest
     best = "cost" if "cost" in out else min(forms, key=lambda f: out[f]["enum_ms"])
-    st = out[best]["res"].stats
+    st = out[best]["stats"]
     tasks = st["tasks"]
     enum_ms = out[best]["enum_ms"]
     A = algo_bytes(st)
     kern_s = out[best]["kern_ms"] / 1e3
     achieved = A / kern_s / 1e9 if kern_s > 0 else 0.0
+    g = job.gs[0]
     line = {
         "metric": "enumeration ms + tasks/sec, batched+cost-ordered vs plain GPU, bit-exact set",
         "value": round(tasks / (enum_ms / 1e3), 1),
@@ -192,27 +249,32 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": w.name, "generator_config": w.config, "seed": w.seed, "gamma": g.gamma,
-                   "min_size": g.min_size, "n": g.n, "window": list(w.window), "form": best},
-        "forms": {f: {"enum_ms": round(o["enum_ms"], 3), "wall_ms": round(o["wall_ms"], 3),
-                      "kernel_ms": round(o["kern_ms"], 3),
-                      "tasks_per_s": round(o["res"].stats["tasks"] / (o["enum_ms"] / 1e3), 1)}
-                  for f, o in out.items()},
+        "config": {"workload": w.name, "generator_config": w.config, "seed": w.seed, "instances": w.instances,
+                   "gamma": g.gamma, "min_size": g.min_size, "n": g.n, "window": list(w.window), "form": best},
+        "forms": table,
         "tasks": tasks,
         "raw_cliques": st["raw_cliques"],
-        "maximal_cliques": len(out[best]["res"].maximal),
+        "maximal_cliques": out[best]["maximal"],
         "maximal_sha256": digest,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC.get(w.name),
                      "algorithmic_bytes": A, "kernel_ms": round(out[best]["kern_ms"], 3)},
     }
-    if "plain" in out:
-        for f in out:
-            if f != "plain":
-                line["forms"][f]["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / out[f]["enum_ms"], 3)
-        if best != "plain":
-            line["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / enum_ms, 3)
+    if "plain" in out and best != "plain":
+        line["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / enum_ms, 3)
     line["setup"] = setup
+    if a.batch_workload >= 0 and a.batch_workload != a.workload:
+        wb = workloads.WORKLOADS[a.batch_workload]
+        bjob = Job(qc, wb)
+        bforms = [f for f in a.batch_forms.split(",") if f]
+        bout, btable, bsetup, bdigest = measure(bjob, bforms, a.steps, a.warmup, a.plain_steps)
+        bbest = "cost" if "cost" in bout else min(bforms, key=lambda f: bout[f]["enum_ms"])
+        line["batch"] = {"workload": wb.name, "instances": wb.instances, "window": list(wb.window), "form": bbest,
+                         "tasks": bout[bbest]["stats"]["tasks"], "ms_per_step": round(bout[bbest]["enum_ms"], 3),
+                         "forms": btable, "maximal_cliques": bout[bbest]["maximal"],
+                         "maximal_sha256": bdigest, "setup": bsetup}
+        if "plain" in bout and bbest != "plain":
+            line["batch"]["speedup_vs_plain"] = round(bout["plain"]["enum_ms"] / bout[bbest]["enum_ms"], 3)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget_s)
     print(json.dumps(line), flush=True)

@@ -299,7 +299,7 @@ def _params(gamma, min_size, scheduling, mode, order, lds_tile, workgroups, task
     p.clique_capacity = int(clique_capacity)
     p.cost_model = cost_model.handle if isinstance(cost_model, CostModel) else cost_model
     rb, re_, md = window if window is not None else (0, -1, -1)
-    p.root_begin, p.root_end, p.max_depth = max(0, int(rb)), max(0, int(re_)), max(0, int(md))
+    p.root_begin, p.root_end, p.max_depth = int(rb), max(0, int(re_)), max(0, int(md))
     p.replays = int(replays)
     p.trace = int(bool(trace))
     p.split_min = int(split_min)

@@ -139,7 +139,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     A.slab_mwords = (uint32_t)I.rw;
     A.stats = d_stats.as<DevStats>();
     A.error = d_err.as<int>();
-    A.g.root_b0 = p.root_begin < 0 ? 0 : p.root_begin;
+    A.g.root_b0 = p.root_begin;
     A.g.root_b1 = p.root_end > 0 ? p.root_end : -1;
     A.g.max_depth = p.max_depth > 0 ? p.max_depth : -1;
 

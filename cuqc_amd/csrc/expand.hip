@@ -1099,7 +1099,9 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
     const int nb = expansions - cov;
     int bend = nb;
     if (root && c.g->root_b1 >= 0 && c.g->root_b1 < bend) bend = (int)c.g->root_b1;
-    const int bbeg = root ? (int)min<long long>(c.g->root_b0, (long long)nb) : 0;
+    // a negative window start counts from the end of the root's branches
+    const long long rb0 = c.g->root_b0 >= 0 ? c.g->root_b0 : max(0ll, (long long)nb + c.g->root_b0);
+    const int bbeg = root ? (int)min<long long>(rb0, (long long)nb) : 0;
 
     // remove one vertex, main.cu:1408-1462; returns true on a member failure
     auto remove_step = [&](bool count) -> bool {
@@ -1141,18 +1143,37 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         return false;
     };
 
+    // exdeg of positions [0, keep) minus their neighbours among positions
+    // [keep, keep + nr): several removals at once
+    auto bulk_remove = [&](int keep, int nr) {
+        if (nr <= 0) return;
+        if (nr < c.W) {
+            for (int k = c.lane; k < nr; k += WAVE) w.adjc[k] = PM(keep + k);
+            wsync();
+            for (int i = c.lane; i < keep; i += WAVE) {
+                const int row = PM(i);
+                int cnt = 0;
+                for (int k = 0; k < nr; k++) cnt += (int)gbit(c.M1, c.W, row, w.adjc[k]);
+                w.pex[i] -= cnt;
+            }
+        } else {
+            clear_mask(c, w.mk0);
+            for (int q = keep + c.lane; q < keep + nr; q += WAVE) set_mask_bit(w.mk0, PM(q));
+            wsync();
+            for (int i = c.lane; i < keep; i += WAVE) w.pex[i] -= rowpop(c, c.M1, PM(i), w.mk0);
+        }
+        wsync();
+    };
+
     int b_lo = 0, b_hi = nb;
     bool count_removes = true;
     if (item) {
         // split item: apply removals 1..b0 at once (the prepare step found no
-        // failure before its range), then run branches [b0, b1)
+        // failure before its range), then run branches [b0, b1); b0 and b1
+        // count from the record the prepare step stored
         const int b0 = (int)ref.b0;
         if (b0 > 0) {
-            clear_mask(c, w.mk0);
-            for (int q = t0 - b0 + c.lane; q < t0; q += WAVE) set_mask_bit(w.mk0, PM(q));
-            wsync();
-            for (int i = c.lane; i < t0 - b0; i += WAVE) w.pex[i] -= rowpop(c, c.M1, PM(i), w.mk0);
-            wsync();
+            bulk_remove(t0 - b0, b0);
             tp = t0 - b0;
             nc -= b0;
         }
@@ -1161,23 +1182,9 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         b_hi = (int)ref.b1;
         count_removes = false;
     } else if (a.queue_mode && a.split_chunk > 0 && (uint32_t)t0 > a.split_min_t && bend - bbeg > a.split_chunk) {
-        // prepare step: keep the record (with lvl2adj) for the items, find
-        // the first failing removal, count the removals' scans, push items
-        unsigned long long slot = 0;
-        if (c.lane == 0) slot = atomicAdd(a.pool_top, (unsigned long long)t0);
-        slot = __shfl(slot, 0, WAVE);
-        if (slot + t0 > a.pool_cap) {
-            if (c.lane == 0) atomicOr(a.error, 1);
-            return;
-        }
-        for (int i = c.lane; i < t0; i += WAVE) {
-            Vrec v;
-            v.id = w.pid[i];
-            v.label = w.plab[i] | (w.pl2[i] << 4);
-            v.indeg = w.pin[i];
-            v.exdeg = w.pex[i];
-            a.pool[slot + i] = v;
-        }
+        // prepare step: find the first failing removal, count the
+        // removals' scans, store the record as it stands before the first
+        // branch of the window (with lvl2adj) and push the items
         // Removal k (branch b = k) takes position t0 - k.  Member i fails at
         // the removal of its need_i-th neighbour counted from the end, with
         // need_i = max(pin + pex - mindeg(nm) + 1, 1); kb = the first such
@@ -1227,6 +1234,29 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         }
         const int hi = min(kb, bend);
         const int ch = a.split_chunk;
+        if (hi <= bbeg) {
+            ph_mark(c, PH_PREPARE);
+            return;
+        }
+        // removals 1..bbeg happen before the window: apply them once here
+        const int pre = bbeg;
+        const int ts = t0 - pre;
+        bulk_remove(ts, pre);
+        unsigned long long slot = 0;
+        if (c.lane == 0) slot = atomicAdd(a.pool_top, (unsigned long long)ts);
+        slot = __shfl(slot, 0, WAVE);
+        if (slot + ts > a.pool_cap) {
+            if (c.lane == 0) atomicOr(a.error, 1);
+            return;
+        }
+        for (int i = c.lane; i < ts; i += WAVE) {
+            Vrec v;
+            v.id = w.pid[i];
+            v.label = w.plab[i] | (w.pl2[i] << 4);
+            v.indeg = w.pin[i];
+            v.exdeg = w.pex[i];
+            a.pool[slot + i] = v;
+        }
         __threadfence();
         for (int b0 = bbeg; b0 < hi; b0 += ch) {
             if (c.lane == 0) {
@@ -1238,13 +1268,13 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                     ri += (unsigned long long)top * a.q_cap;
                     TaskRef r;
                     r.off = slot;
-                    r.t = (uint32_t)t0;
+                    r.t = (uint32_t)ts;
                     r.depth = ref.depth;
                     r.inst = ref.inst;
                     r.bound = (uint64_t)c.slot;  // the task the item belongs to
                     if (a.trace_parent) a.trace_parent[ri] = -2;
-                    r.b0 = (uint32_t)b0;
-                    r.b1 = (uint32_t)min(b0 + ch, hi);
+                    r.b0 = (uint32_t)(b0 - pre);
+                    r.b1 = (uint32_t)(min(b0 + ch, hi) - pre);
                     atomicAdd((unsigned long long*)a.pending, 1ull);
                     a.q_refs[ri] = r;
                     __threadfence();

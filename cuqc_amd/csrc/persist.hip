@@ -131,7 +131,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     for (int i = 0; i < count; i++) {
         const Instance& I = *insts[i];
         graphs[i] = I.dg;
-        graphs[i].root_b0 = params[i].root_begin < 0 ? 0 : params[i].root_begin;
+        graphs[i].root_b0 = params[i].root_begin;
         graphs[i].root_b1 = params[i].root_end > 0 ? params[i].root_end : -1;
         graphs[i].max_depth = params[i].max_depth > 0 ? params[i].max_depth : -1;
         if (I.R == 0) continue;

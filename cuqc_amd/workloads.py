@@ -1,5 +1,6 @@
 """The five benchmark workloads: generator config, seed, mining parameters and
-the search window each one is measured on.
+the search window each one is measured on (a negative root_begin counts
+branches back from the last one).
 
 Exact maximal-quasi-clique enumeration with the reference's algorithm is
 exponential in the size of dense regions; on configs 1-4 the full search
@@ -24,6 +25,7 @@ class Workload:
     root_end: int = -1      # -1: all root branches
     max_depth: int = -1     # -1: no depth limit
     note: str = ""
+    instances: int = 1      # > 1: a batch of generated instances mined together
 
     @property
     def window(self) -> tuple[int, int, int]:
@@ -38,7 +40,8 @@ WORKLOADS: dict[int, Workload] = {
     0: Workload("er200-planted", 0, 1, note="full search"),
     1: Workload("coexpr8k", 1, 2, 0, -1, 1, note="all 4163 root branches, depth <= 1"),
     2: Workload("chunglu100k", 2, 3, note="window pending calibration"),
-    3: Workload("batch256", 3, 4, note="per-instance windows"),
+    3: Workload("batch256", 3, 4, -32, -1, 2, instances=256,
+                note="256 instances, each: its last 32 root branches, depth <= 2"),
     4: Workload("chunglu1m", 4, 5, note="window pending calibration"),
 }
 
@@ -46,3 +49,8 @@ WORKLOADS: dict[int, Workload] = {
 def graph(w: Workload, instance: int = 0):
     from . import Graph
     return Graph.generate(w.config, w.seed, instance)
+
+
+def graphs(w: Workload) -> list:
+    """Every instance of the workload (one graph unless it is a batch)."""
+    return [graph(w, i) for i in range(w.instances)]

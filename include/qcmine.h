@@ -64,8 +64,9 @@ typedef struct qc_params {
     uint64_t task_capacity;  /* vertex slots per task pool; 0 = auto */
     uint64_t clique_capacity;/* raw-clique vertex slots; 0 = auto */
     const void* cost_model;  /* qc_model*, required when order == QC_ORDER_COST */
-    int64_t root_begin;      /* search window: root branches [root_begin, root_end) in */
-    int64_t root_end;        /*   the reference's branch order (main.cu:1094); end <= 0: all */
+    int64_t root_begin;      /* search window: root branches [root_begin, root_end) in the */
+    int64_t root_end;        /*   reference's branch order (main.cu:1094); end <= 0: to the last;
+                                begin < 0: that many branches before the last one's end */
     int32_t replays;         /* persistent mode: launch the captured hipGraph this many times and
                                 require identical counters from every replay; <= 1: once */
     int32_t trace;           /* persistent mode: record the cost-model features of every expanded

@@ -112,7 +112,8 @@ struct Oracle {
     bool faithful = false;
     bool trace = false;
     // search window: root branches b in [rb0, rb1) (b = j - covered, the
-    // reference's loop order) and tasks of depth <= max_depth are expanded
+    // reference's loop order; rb0 < 0 starts that many branches before the
+    // end) and tasks of depth <= max_depth are expanded
     long rb0 = 0, rb1 = -1;
     int max_depth = -1;
     int cur_depth = 0;
@@ -486,12 +487,14 @@ struct Oracle {
         std::vector<Vtx> vs;
         const bool root = cur_depth == 0;
         if (root) st.root_branches = expansions - cov;
+        // a negative window start counts from the end of the root's branches
+        const long b0 = rb0 >= 0 ? rb0 : std::max(0L, (long)(expansions - cov) + rb0);
         for (int j = cov; j < expansions; j++) {
             if (root && rb1 >= 0 && j - cov >= rb1) break;
             if (j != cov) {
                 if (remove_one_vertex(rec.data(), t, nc, nm)) break;
             }
-            if (root && j - cov < rb0) continue;
+            if (root && j - cov < b0) continue;
             vs.resize(t);
             int m = nm, c = nc, tt = t;
             for (int i = 0; i < m; i++) vs[i] = rec[i];
@@ -791,8 +794,8 @@ orc_result* orc_mine(int n, const uint64_t* off, const int* adj, const uint64_t*
     return orc_mine_window(n, off, adj, off2, adj2, gamma, min_size, faithful, trace, 0, -1, -1);
 }
 
-// Same, restricted to root branches [rb0, rb1) (rb1 < 0: all) and to tasks
-// of depth <= max_depth (max_depth < 0: all).
+// Same, restricted to root branches [rb0, rb1) (rb1 < 0: all; rb0 < 0: the
+// last -rb0 branches) and to tasks of depth <= max_depth (max_depth < 0: all).
 orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                             double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth) {
     Oracle o;
@@ -805,7 +808,7 @@ orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const ui
     o.ms = min_size;
     o.faithful = faithful != 0;
     o.trace = trace != 0;
-    o.rb0 = rb0 < 0 ? 0 : rb0;
+    o.rb0 = rb0;
     o.rb1 = rb1;
     o.max_depth = max_depth;
     o.build_tables();

@@ -119,6 +119,34 @@ def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dic
             "maximal_sha256": digest(mx)}
 
 
+def _instance_result(args):
+    wl, inst = args
+    from cuqc_amd import workloads
+    w = workloads.WORKLOADS[wl]
+    g = workloads.graph(w, inst)
+    off, adj = g.csr()
+    r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=w.root_begin, root_end=w.root_end,
+                    max_depth=w.max_depth)
+    mx = oracle.nonmax(r["raw"])
+    return {"workload": w.name, "instance": inst, "checksum": hex(g.checksum()), "window": list(w.window),
+            "gamma": g.gamma, "min_size": g.min_size, "root_branches": r["stats"]["root_branches"],
+            "stats": {k: r["stats"][k] for k in STATS}, "raw_sha256": digest(oracle.canon(r["raw"])),
+            "maximal": len(mx), "maximal_sha256": digest(mx)}
+
+
+def batch_results(wl: int, jobs: int = 8) -> dict:
+    """Per-instance oracle results of a batch workload (one process per
+    instance)."""
+    from cuqc_amd import workloads
+    w = workloads.WORKLOADS[wl]
+    out = {}
+    with ProcessPoolExecutor(int(os.environ.get("GOLDEN_JOBS", jobs))) as ex:
+        for r in ex.map(_instance_result, [(wl, i) for i in range(w.instances)]):
+            out[f"{wl}:{r['instance']}"] = r
+            print(f"[golden] {wl}:{r['instance']} tasks {r['stats']['tasks']}", flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--windows", default="")
@@ -132,9 +160,13 @@ def main():
     if a.windows:
         path = os.path.join(HERE, "windows.json")
         cur = json.load(open(path)) if os.path.exists(path) else {}
+        from cuqc_amd import workloads
         for wl in [int(x) for x in a.windows.split(",")]:
-            cur[str(wl)] = window_result(wl)
-            print(json.dumps(cur[str(wl)]), flush=True)
+            if workloads.WORKLOADS[wl].instances > 1:
+                cur.update(batch_results(wl))
+            else:
+                cur[str(wl)] = window_result(wl)
+                print(json.dumps(cur[str(wl)]), flush=True)
         with open(path, "w") as f:
             json.dump(cur, f, indent=1)
 

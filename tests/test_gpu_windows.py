@@ -34,7 +34,11 @@ def _check(res, gold):
     assert _digest(res.maximal_sorted_tuples()) == gold["maximal_sha256"]
 
 
-@pytest.mark.parametrize("key", sorted(WINDOWS))
+SINGLE = sorted(k for k in WINDOWS if ":" not in k)
+BATCHES = sorted({int(k.split(":")[0]) for k in WINDOWS if ":" in k})
+
+
+@pytest.mark.parametrize("key", SINGLE)
 @pytest.mark.parametrize("order", ["fifo", "cost"])
 def test_workload_window_persistent(key, order):
     gold = WINDOWS[key]
@@ -46,3 +50,36 @@ def test_workload_window_persistent(key, order):
     sys.stdout.flush()
     res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", window=w.window, order=order, cost_model=model)
     _check(res, gold)
+
+
+def _batch(wl):
+    w = workloads.WORKLOADS[wl]
+    golds = [WINDOWS[f"{wl}:{i}"] for i in range(w.instances)]
+    gs = workloads.graphs(w)
+    for g, gold in zip(gs, golds):
+        assert hex(g.checksum()) == gold["checksum"]
+    return w, gs, golds
+
+
+@pytest.mark.parametrize("wl", BATCHES)
+@pytest.mark.parametrize("order", ["fifo", "cost"])
+def test_batch_workload_one_launch(wl, order):
+    """Every instance of the batch in one persistent launch (hipGraph, two
+    replays) equals its own oracle result."""
+    w, gs, golds = _batch(wl)
+    model = qc.CostModel.create(seed=3) if order == "cost" else None
+    print(f"[gpu] batch {w.name} ({len(gs)} instances) {order}", flush=True)
+    res = qc.mine_batch(gs, [g.gamma for g in gs], [g.min_size for g in gs], windows=[w.window] * len(gs),
+                        order=order, cost_model=model, replays=2)
+    for r, gold in zip(res, golds):
+        _check(r, gold)
+
+
+@pytest.mark.parametrize("wl", BATCHES)
+def test_batch_workload_plain_subset(wl):
+    """Every 16th instance on its own in the plain level-synchronous form."""
+    w, gs, golds = _batch(wl)
+    for i in range(0, len(gs), 16):
+        print(f"[gpu] batch {w.name} instance {i} plain", flush=True)
+        r = qc.mine(gs[i], gs[i].gamma, gs[i].min_size, 0, mode="plain", window=w.window)
+        _check(r, golds[i])

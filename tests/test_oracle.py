@@ -229,3 +229,15 @@ def test_golden_small_vectors():
         r = oracle.mine(off, adj, case["gamma"], case["min_size"])
         assert r["stats"]["tasks"] == case["tasks"], case["name"]
         assert [list(s) for s in oracle.nonmax(r["raw"])] == case["maximal"], case["name"]
+
+
+@pytest.mark.parametrize("name,build,gamma,ms", SMALL_CASES[:4], ids=[c[0] for c in SMALL_CASES[:4]])
+def test_window_from_the_end(name, build, gamma, ms):
+    """A negative window start counts branches back from the last one."""
+    off, adj = build()
+    nb = oracle.mine(off, adj, gamma, ms, root_begin=0, root_end=0)["stats"]["root_branches"]
+    k = max(1, nb // 3)
+    a = oracle.mine(off, adj, gamma, ms, root_begin=-k, root_end=-1)
+    b = oracle.mine(off, adj, gamma, ms, root_begin=nb - k, root_end=-1)
+    assert oracle.canon(a["raw"]) == oracle.canon(b["raw"])
+    assert a["stats"] == b["stats"]
