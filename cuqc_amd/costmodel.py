@@ -63,9 +63,13 @@ def loss_and_grad(z: np.ndarray, t: np.ndarray, p: np.ndarray):
     return loss, g
 
 
-def subtree_sizes(parent: np.ndarray) -> np.ndarray:
-    """Tasks per subtree from parent links (children after their parent)."""
+def subtree_sizes(parent: np.ndarray, beyond: np.ndarray | None = None) -> np.ndarray:
+    """Task records per subtree from parent links (children after their
+    parent); ``beyond`` adds each task's children past a depth window, which
+    are created but not expanded."""
     sub = np.ones(len(parent), np.int64)
+    if beyond is not None:
+        sub += np.asarray(beyond, np.int64)
     for i in range(len(parent) - 1, -1, -1):
         if parent[i] >= 0:
             sub[parent[i]] += sub[i]

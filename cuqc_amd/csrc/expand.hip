@@ -877,6 +877,7 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
             c.st->children++;
             c.st->sum_tw += tc;
             c.st->beyond++;
+            if (a.trace_beyond && c.slot >= 0) atomicAdd(&a.trace_beyond[c.slot], 1u);
         }
         return;
     }

@@ -30,7 +30,7 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
                               uint64_t root_slots, TaskRef* q_refs, unsigned int* q_ready, uint64_t q_cap, int nb,
                               Vrec* pool, unsigned long long* ctr, StreamCtr* clq_ctr, DevStats* stats,
                               DevStats* inst_stats, uint32_t n_inst, int* error, const float* __restrict__ root_feat,
-                              float* trace_feat, long long* trace_parent) {
+                              float* trace_feat, long long* trace_parent, unsigned int* trace_beyond) {
     const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t top = (uint64_t)(nb - 1) * q_cap;
@@ -39,6 +39,7 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
     if (trace_feat) {
         for (uint64_t i = tid; i < (uint64_t)nroots * 12; i += nt) trace_feat[i] = root_feat[i];
         for (uint64_t i = tid; i < nroots; i += nt) trace_parent[i] = -1;
+        for (uint64_t i = tid; i < q_cap; i += nt) trace_beyond[i] = 0;
     }
     for (uint64_t i = tid; i < root_slots; i += nt) pool[i] = root_verts[i];
     for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -209,7 +210,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     }
 
     const bool trace = p.trace != 0;
-    DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_rfeat, d_cmw, d_cmn, d_phase;
+    DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_tbey, d_rfeat, d_cmw, d_cmn, d_phase;
     if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
     if ((rc = ensure(ws.ctr, 10 * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
         (rc = ensure(ws.stats, sizeof(DevStats))) || (rc = ensure(ws.err, sizeof(int))) ||
@@ -253,7 +254,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
             (rc = ensure(ws.ready, nb * q_cap * 4)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
             (rc = ensure(ws.clq_verts, clq_slots * 4)))
             return rc;
-        if (trace && ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8))))
+        if (trace && ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8)) ||
+                      (rc = d_tbey.alloc(q_cap * 4))))
             return rc;
         ws.pool_cap = std::max(ws.pool_cap, pool_cap);
         ws.q_cap = std::max<uint64_t>(ws.q_cap, nb * q_cap);
@@ -298,6 +300,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.pool_cap = pool_cap;
         A.trace_feat = trace ? d_tfeat.as<float>() : nullptr;
         A.trace_parent = trace ? (long long*)d_tpar.p : nullptr;
+        A.trace_beyond = trace ? d_tbey.as<unsigned int>() : nullptr;
         A.phase = phases ? d_phase.as<unsigned long long>() : nullptr;
         QC_HIP(hipMemcpyAsync(ws.args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
         QC_HIP(hipStreamSynchronize(s));
@@ -313,7 +316,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
                            d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), ws.q.as<TaskRef>(),
                            ws.ready.as<unsigned int>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
                            ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
-                           (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent);
+                           (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent,
+                           A.trace_beyond);
         hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, lds_b, s);
         hipError_t ce = hipStreamEndCapture(s, &graph);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -442,10 +446,15 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         std::vector<float> tf(tail * 12);
         std::vector<long long> tp(tail);
         std::vector<TaskRef> qr(tail);
+        std::vector<unsigned int> tb(tail);
         QC_HIP(hipMemcpy(tf.data(), d_tfeat.p, tail * 12 * sizeof(float), hipMemcpyDeviceToHost));
         QC_HIP(hipMemcpy(tp.data(), d_tpar.p, tail * 8, hipMemcpyDeviceToHost));
         QC_HIP(hipMemcpy(qr.data(), ws.q.p, tail * sizeof(TaskRef), hipMemcpyDeviceToHost));
+        QC_HIP(hipMemcpy(tb.data(), d_tbey.p, tail * 4, hipMemcpyDeviceToHost));
+        // records per subtree: the task, its children past the depth window
+        // and its expanded children's subtrees
         std::vector<int64_t> sub(tail, 1), row(tail, -1);
+        for (uint64_t i = 0; i < tail; i++) sub[i] += tb[i];
         for (uint64_t i = tail; i-- > 0;)
             if (tp[i] >= 0) sub[tp[i]] += sub[i];
         for (uint64_t i = 0; i < tail; i++) {

@@ -160,6 +160,7 @@ struct ExpandArgs {
     // (-1 root, -2 split item)
     float* trace_feat;
     long long* trace_parent;
+    unsigned int* trace_beyond;  // per slot: children past the depth window
 
     int32_t n_buckets;               // 1 = FIFO
     int32_t pad2;

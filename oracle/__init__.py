@@ -49,6 +49,8 @@ def lib() -> C.CDLL:
         L.orc_trace_feats.argtypes = [P]
         L.orc_trace_parent.restype = C.POINTER(C.c_int64)
         L.orc_trace_parent.argtypes = [P]
+        L.orc_trace_beyond.restype = C.POINTER(C.c_int64)
+        L.orc_trace_beyond.argtypes = [P]
         L.orc_free.argtypes = [P]
         L.orc_nonmax.restype = P
         L.orc_nonmax.argtypes = [C.c_uint64, P, P]
@@ -99,6 +101,8 @@ def mine(off, adj, gamma: float, min_size: int, *, faithful: bool = False, trace
             out["trace_feats"] = np.ctypeslib.as_array(L.orc_trace_feats(h), shape=(n, 12)).copy() if n else \
                 np.zeros((0, 12), np.float32)
             out["trace_parent"] = np.ctypeslib.as_array(L.orc_trace_parent(h), shape=(n,)).copy() if n else \
+                np.zeros(0, np.int64)
+            out["trace_beyond"] = np.ctypeslib.as_array(L.orc_trace_beyond(h), shape=(n,)).copy() if n else \
                 np.zeros(0, np.int64)
         return out
     finally:

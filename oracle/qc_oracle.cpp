@@ -78,6 +78,7 @@ struct Stats {
 struct TraceRow {
     float f[12];
     int64_t parent;
+    int64_t beyond;  // children past the depth window (counted, not expanded)
 };
 
 // Q order of main.cu:2059-2104: label rank 1 < 2 < 0 < 3 < other, then
@@ -738,6 +739,7 @@ struct Oracle {
             if ((uint64_t)it.depth > st.max_depth) st.max_depth = it.depth;
             if (max_depth >= 0 && it.depth > max_depth) {
                 st.beyond++;
+                if (trace && it.tid >= 0) trace_rows[it.tid].beyond++;
                 continue;
             }
             int64_t my = -1;
@@ -745,6 +747,7 @@ struct Oracle {
                 TraceRow r;
                 memcpy(r.f, it.f, sizeof(r.f));
                 r.parent = it.tid;
+                r.beyond = 0;
                 my = (int64_t)trace_rows.size();
                 trace_rows.push_back(r);
             }
@@ -783,6 +786,7 @@ struct orc_result {
     orc_stats st;
     std::vector<float> trace_f;
     std::vector<int64_t> trace_parent;
+    std::vector<int64_t> trace_beyond;
 };
 
 // Mine a graph to completion.  off2/adj2 may be NULL (N2 derived from N).
@@ -821,9 +825,11 @@ orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const ui
     if (o.trace) {
         r->trace_f.resize(o.trace_rows.size() * 12);
         r->trace_parent.resize(o.trace_rows.size());
+        r->trace_beyond.resize(o.trace_rows.size());
         for (size_t i = 0; i < o.trace_rows.size(); i++) {
             memcpy(&r->trace_f[i * 12], o.trace_rows[i].f, 12 * sizeof(float));
             r->trace_parent[i] = o.trace_rows[i].parent;
+            r->trace_beyond[i] = o.trace_rows[i].beyond;
         }
     }
     return r;
@@ -836,6 +842,7 @@ void orc_get_stats(const orc_result* r, orc_stats* s) { *s = r->st; }
 uint64_t orc_trace_rows(const orc_result* r) { return r->trace_parent.size(); }
 const float* orc_trace_feats(const orc_result* r) { return r->trace_f.data(); }
 const int64_t* orc_trace_parent(const orc_result* r) { return r->trace_parent.data(); }
+const int64_t* orc_trace_beyond(const orc_result* r) { return r->trace_beyond.data(); }
 void orc_free(orc_result* r) { delete r; }
 
 // RemoveNonMax semantics of main.cu:4569-4961: each set is sorted ascending,

@@ -57,6 +57,21 @@ def test_subtree_sizes():
     # 0 -> (1 -> 3, 2), 4 is a second root
     parent = np.array([-1, 0, 0, 1, -1])
     assert list(cm.subtree_sizes(parent)) == [4, 2, 1, 1, 1]
+    # children past a depth window count as leaves of their parent
+    assert list(cm.subtree_sizes(parent, [0, 2, 5, 0, 1])) == [11, 4, 6, 1, 2]
+
+
+def test_oracle_trace_counts_children_past_the_window():
+    import cuqc_amd as qc
+    import oracle
+    g = qc.Graph.generate(0, 1)
+    off, adj = g.csr()
+    r = oracle.mine(off, adj, g.gamma, g.min_size, trace=True, max_depth=2)
+    assert int(np.sum(r["trace_beyond"])) == r["stats"]["beyond"] > 0
+    depth = r["trace_feats"][:, 8]
+    assert np.all(r["trace_beyond"][depth < 2] == 0)
+    sub = cm.subtree_sizes(r["trace_parent"], r["trace_beyond"])
+    assert sub[0] == r["stats"]["tasks"] + r["stats"]["beyond"]
 
 
 def test_model_file_round_trip(tmp_path):

@@ -143,19 +143,19 @@ def _trace_rows(feats, subtree):
     return sorted((tuple(float(x) for x in f), int(s)) for f, s in zip(feats, subtree))
 
 
-@pytest.mark.parametrize("chunk", [-1, 4])
-def test_trace_matches_oracle(cfg0, chunk):
-    """The GPU task trace (features + subtree sizes) equals the oracle's."""
+@pytest.mark.parametrize("chunk,window", [(-1, None), (4, None), (-1, (0, -1, 2)), (3, (5, -1, 3))])
+def test_trace_matches_oracle(cfg0, chunk, window):
+    """The GPU task trace (features + subtree sizes, children past a depth
+    window included) equals the oracle's."""
+    from cuqc_amd import costmodel as cm
     g, _ = cfg0
     off, adj = g.csr()
-    ref = oracle.mine(off, adj, g.gamma, g.min_size, trace=True)
-    sub = np.ones(len(ref["trace_parent"]), np.int64)
-    for i in range(len(sub) - 1, -1, -1):
-        p = ref["trace_parent"][i]
-        if p >= 0:
-            sub[p] += sub[i]
-    _say(f"[gpu] trace chunk={chunk}")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, trace=True)
+    w = window or (0, -1, -1)
+    ref = oracle.mine(off, adj, g.gamma, g.min_size, trace=True, root_begin=w[0], root_end=w[1], max_depth=w[2])
+    sub = cm.subtree_sizes(ref["trace_parent"], ref["trace_beyond"])
+    _say(f"[gpu] trace chunk={chunk} window={window}")
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, trace=True,
+                  window=window)
     assert len(res.trace_subtree) == ref["stats"]["tasks"]
     assert _trace_rows(res.trace_feats, res.trace_subtree) == _trace_rows(ref["trace_feats"], sub)
 
