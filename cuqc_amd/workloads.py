@@ -39,10 +39,10 @@ class Workload:
 WORKLOADS: dict[int, Workload] = {
     0: Workload("er200-planted", 0, 1, note="full search"),
     1: Workload("coexpr8k", 1, 2, 0, -1, 1, note="all 4163 root branches, depth <= 1"),
-    2: Workload("chunglu100k", 2, 3, note="window pending calibration"),
+    2: Workload("chunglu100k", 2, 3, -32, -1, 2, note="last 32 of 13087 root branches, depth <= 2"),
     3: Workload("batch256", 3, 4, -32, -1, 2, instances=256,
                 note="256 instances, each: its last 32 root branches, depth <= 2"),
-    4: Workload("chunglu1m", 4, 5, note="window pending calibration"),
+    4: Workload("chunglu1m", 4, 5, -8, -1, 2, note="last 8 of 66981 root branches, depth <= 2"),
 }
 
 

@@ -101,7 +101,8 @@ def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dic
     g = workloads.graph(w, inst)
     off, adj = g.csr()
     nb = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=0, root_end=0)["stats"]["root_branches"]
-    b0 = max(0, w.root_begin)
+    # a negative window start counts back from the last root branch
+    b0 = w.root_begin if w.root_begin >= 0 else max(0, nb + w.root_begin)
     b1 = nb if w.root_end < 0 else min(nb, w.root_end)
     cuts = sorted(set([b0] + [b0 + (b1 - b0) * i // parts for i in range(1, parts)] + [b1]))
     # the first part also returns the root's own counters
