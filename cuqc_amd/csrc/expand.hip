@@ -468,10 +468,14 @@ __device__ inline bool members_fail(Ctx& c, int m, int ub, int lb, int med) {
 __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb,
                                                               int& med, int nind, bool map_empty) {
     WS& w = c.w;
+    ph_mark(c, PH_PRUNE);
     int V = indeg_dist(c, nind);
     if (V < 0) sort_desc_int(c, w.cind, nind);
-    if (lu_bounds(c, m, nind, ub, lb, med, V)) return true;
-    if (members_fail(c, m, ub, lb, med)) return true;
+    if (lu_bounds(c, m, nind, ub, lb, med, V) || members_fail(c, m, ub, lb, med)) {
+        ph_mark(c, PH_LU);
+        return true;
+    }
+    ph_mark(c, PH_LU);
     // classify candidates, stable
     int nrem = 0, nrmv = 0;
     for (int base = m; base < tc; base += WAVE) {
@@ -485,6 +489,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
         nrmv += __popcll(bx);
     }
     wsync();
+    ph_mark(c, PH_COMPACT);
     while (nrem > 0 && nrmv > 0) {
         // degree update, main.cu:1775-1807
         {
@@ -530,6 +535,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
         }
 #undef UPD_IDX
         wsync();
+        ph_mark(c, PH_DEGUPD);
         // indegrees of remaining candidates valid under the previous bounds
         int nv = 0;
         for (int base = 0; base < nrem; base += WAVE) {
@@ -546,10 +552,14 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             nv += __popcll(bo);
         }
         wsync();
+        ph_mark(c, PH_COMPACT);
         V = indeg_dist(c, nv);
         if (V < 0) sort_desc_int(c, w.cind, nv);
-        if (lu_bounds(c, m, nv, ub, lb, med, V)) return true;
-        if (members_fail(c, m, ub, lb, med)) return true;
+        if (lu_bounds(c, m, nv, ub, lb, med, V) || members_fail(c, m, ub, lb, med)) {
+            ph_mark(c, PH_LU);
+            return true;
+        }
+        ph_mark(c, PH_LU);
         // re-classify the remaining set under the new bounds (in place)
         int kept = 0;
         nrmv = 0;
@@ -595,6 +605,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
     }
     tc = tc - nc + nrem;
     nc = nrem;
+    ph_mark(c, PH_COMPACT);
     return false;
 }
 
@@ -1385,7 +1396,9 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 skip = true;
                 break;
             }
-            if (critical_prep(c, tc, cc, m, lb, nind, map_empty) == 2) {
+            const int crit = critical_prep(c, tc, cc, m, lb, nind, map_empty);
+            ph_mark(c, PH_CRIT);
+            if (crit == 2) {
                 skip = true;
                 break;
             }

@@ -184,5 +184,9 @@ enum PhaseId {
     PH_CHILD,         // child sort and write
     PH_PREPARE,       // split preparation
     PH_IDLE,          // persistent pop loop: waiting for work
+    PH_LU,            // candidate indegree tables, LU bounds, member checks
+    PH_DEGUPD,        // degree-pruning exdeg updates
+    PH_COMPACT,       // degree-pruning classification and compaction
+    PH_CRIT,          // critical-vertex step
     PH_COUNT
 };
