@@ -96,6 +96,9 @@ def load_library() -> C.CDLL:
         "qc_result_stats": (C.c_int, [P, C.POINTER(_Stats)]),
         "qc_result_trace": (C.c_int, [P, u64p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_int64)),
                                       C.POINTER(C.POINTER(C.c_int64))]),
+        "qc_convert_edge_to_adj": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32]),
+        "qc_convert_rm_empty": (C.c_int, [C.c_char_p, C.c_char_p]),
+        "qc_convert_adj_to_ser": (C.c_int, [C.c_char_p, C.c_char_p]),
         "qc_result_write": (C.c_int, [P, C.c_char_p]),
         "qc_result_write_raw": (C.c_int, [P, C.c_char_p]),
         "qc_result_free": (None, [P]),
@@ -416,6 +419,21 @@ class CostModel:
                 load_library().qc_model_free(self._h)
         except Exception:
             pass
+
+
+def convert_edge_to_adj(in_path, out_path, duplicate: bool = False) -> None:
+    """edgeToAdj.py: edge list -> adjacency list text."""
+    _check(load_library().qc_convert_edge_to_adj(os.fsencode(in_path), os.fsencode(out_path), int(bool(duplicate))))
+
+
+def convert_rm_empty(in_path, out_path) -> None:
+    """rmEmptyVert.py: drop empty adjacency lines and renumber."""
+    _check(load_library().qc_convert_rm_empty(os.fsencode(in_path), os.fsencode(out_path)))
+
+
+def convert_adj_to_ser(in_path, out_path) -> None:
+    """adjToSer.cpp: adjacency list text -> serialized graph with N2 lists."""
+    _check(load_library().qc_convert_adj_to_ser(os.fsencode(in_path), os.fsencode(out_path)))
 
 
 def degree_tables(gamma: float, n: int) -> tuple[np.ndarray, np.ndarray]:

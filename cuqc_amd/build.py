@@ -3,8 +3,9 @@
     python -m cuqc_amd.build            # compile what changed
     python -m cuqc_amd.build --force    # recompile everything
 
-Objects go to cuqc_amd/_build/, the library to cuqc_amd/libqcmine.so and the
-CLI to cuqc_amd/qcmine (rpath $ORIGIN).
+Objects go to cuqc_amd/_build/, the library to cuqc_amd/libqcmine.so, the
+CLI to cuqc_amd/qcmine and the dataset converter CLI to cuqc_amd/qcconv
+(rpath $ORIGIN).
 """
 from __future__ import annotations
 
@@ -20,12 +21,13 @@ SRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libqcmine.so"
 CLI = PKG / "qcmine"
+CONV = PKG / "qcconv"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["expand.hip", "root.hip", "driver.hip", "nonmax.hip", "persist.hip", "mlp.hip", "capi.hip"]
-CXX_SOURCES = ["graph_io.cpp", "gen.cpp"]
+CXX_SOURCES = ["graph_io.cpp", "gen.cpp", "convert.cpp"]
 HEADERS = ["qc_host.h", "qc_device.h", "qc_internal.h"]
 
 HIP_FLAGS = [
@@ -77,6 +79,10 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
     main_src = SRC / "qcmine_main.cpp"
     if force or not CLI.exists() or CLI.stat().st_mtime < max(LIB.stat().st_mtime, main_src.stat().st_mtime):
         _run(["g++", *CXX_FLAGS, str(main_src), "-o", str(CLI), f"-L{PKG}", "-lqcmine",
+              "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"])
+    conv_src = SRC / "qcconv_main.cpp"
+    if force or not CONV.exists() or CONV.stat().st_mtime < max(LIB.stat().st_mtime, conv_src.stat().st_mtime):
+        _run(["g++", *CXX_FLAGS, str(conv_src), "-o", str(CONV), f"-L{PKG}", "-lqcmine",
               "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"])
     return LIB
 

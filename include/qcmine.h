@@ -217,6 +217,20 @@ void qc_model_free(qc_model* m);
  * the reference's double arithmetic. */
 int qc_degree_tables(double gamma, int32_t n_mindeg, int32_t* mindeg, int32_t n_ubq, int32_t* ubq);
 
+/* ---- dataset converters (the reference's preprocessing tools) ---- */
+
+/* edgeToAdj.py: "u v" lines -> one adjacency line per vertex 0..max id,
+ * neighbours in input order; duplicate != 0 also appends u to v's line. */
+int qc_convert_edge_to_adj(const char* in_path, const char* out_path, int32_t duplicate);
+
+/* rmEmptyVert.py: drop empty adjacency lines and renumber the others in line
+ * order (a value naming an empty line is QC_EINVAL). */
+int qc_convert_rm_empty(const char* in_path, const char* out_path);
+
+/* adjToSer.cpp (main, CPU_Graph, write_serialized): adjacency text -> the
+ * 7-record serialized graph with N2 lists, byte for byte. */
+int qc_convert_adj_to_ser(const char* in_path, const char* out_path);
+
 /* ---- misc ------------------------------------------------------------ */
 
 const char* qc_last_error(void);
