@@ -166,10 +166,11 @@ def main():
             if workloads.WORKLOADS[wl].instances > 1:
                 cur.update(batch_results(wl))
             else:
-                cur[str(wl)] = window_result(wl)
+                cur[str(wl)] = window_result(wl, parts=int(os.environ.get("GOLDEN_PARTS", 64)))
                 print(json.dumps(cur[str(wl)]), flush=True)
-        with open(path, "w") as f:
-            json.dump(cur, f, indent=1)
+            # saved after every workload
+            with open(path, "w") as f:
+                json.dump(cur, f, indent=1)
 
 
 if __name__ == "__main__":
