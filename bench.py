@@ -30,11 +30,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the dominant kernel (expand_persistent), from the
-# rocprofv3 FETCH_SIZE and WRITE_SIZE passes recorded under profiles/ for the
-# default schedule, each scaled by the counter's reading for a device copy
-# of known size (FETCH x2.0, WRITE x1.0); None where not measured.
-TRAFFIC: dict = {}
+# HBM bytes per launch of the dominant kernel (expand_persistent) in the
+# headline form, from one rocprofv3 FETCH_SIZE and one WRITE_SIZE pass, each
+# scaled by the counter's reading for a 256 MiB device copy (FETCH x2.0,
+# WRITE x1.0; profiles/hbm_traffic_coexpr8k_depth1.md).  Measured for the
+# schedule lds_tile 256 / split_chunk 32 / split_min 512 / lds_wide 512;
+# None where not measured.
+TRAFFIC: dict = {"coexpr8k": 96716122178}
 
 
 def _log(msg: str) -> None:
@@ -134,6 +136,9 @@ class Job:
                     maximal=sum(len(r.maximal) for r in res))
 
 
+SCHEDULE_CACHE = None
+
+
 def setup_forms(job: Job, forms):
     """Untimed set-up: the autotuned schedule and the trained cost model."""
     from cuqc_amd import autotune, costmodel
@@ -141,7 +146,7 @@ def setup_forms(job: Job, forms):
     sched = None
     if "tuned" in forms or "cost" in forms:
         t0 = time.perf_counter()
-        sched = autotune.tune(job.gs, job.gammas, job.min_sizes, job.windows, reps=1, log=_log)
+        sched = autotune.tune(job.gs, job.gammas, job.min_sizes, job.windows, reps=1, log=_log, cache=SCHEDULE_CACHE)
         info["autotune"] = dict(sched.kwargs(), trials=len(sched.trials), seconds=round(time.perf_counter() - t0, 2))
         kws["tuned"] = dict(mode="persistent", **sched.kwargs())
     if "cost" in forms:
@@ -203,6 +208,8 @@ def main() -> int:
     ap.add_argument("--batch-forms", default="plain,persistent,cost")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule-cache", default=None,
+                    help="JSON file of autotuned schedules to reuse (and extend)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (HBM counter calibration)")
     a = ap.parse_args()
@@ -222,6 +229,8 @@ def main() -> int:
     import cuqc_amd as qc
     from cuqc_amd import workloads
 
+    global SCHEDULE_CACHE
+    SCHEDULE_CACHE = a.schedule_cache
     w = workloads.WORKLOADS[a.workload]
     job = Job(qc, w)
     forms = [f for f in a.forms.split(",") if f]
