@@ -1,10 +1,14 @@
 // capi.hip -- the C ABI of include/qcmine.h.
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "qc_internal.h"
 
@@ -21,6 +25,51 @@ int enumerate_persistent(const Instance* const* inst, const qc_params* params, i
 using namespace qc;
 
 namespace {
+
+// Roots and device tables of a batch, built by up to 16 host threads, each
+// with a HIP stream of its own (the host filter / peel / Q-sort of one
+// instance overlaps the device work of the others).
+int build_instances(const qc_graph* const* graphs, const qc_params* params, int count,
+                    std::vector<std::unique_ptr<Instance>>& insts) {
+    int dev = 0;
+    QC_HIP(hipGetDevice(&dev));
+    std::vector<int> rcs(count, QC_OK);
+    std::vector<std::string> errs(count);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        hipStream_t ts = nullptr;
+        if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&ts) != hipSuccess) {
+            for (int i = next++; i < count; i = next++) {
+                rcs[i] = QC_EDEVICE;
+                errs[i] = "cannot create a HIP stream for root construction";
+            }
+            return;
+        }
+        for (int i = next++; i < count; i = next++) {
+            try {
+                insts[i].reset(new Instance);
+                rcs[i] = build_instance(graphs[i], params[i].gamma, params[i].min_size, *insts[i], ts);
+            } catch (const std::bad_alloc&) {
+                set_error("out of host memory");
+                rcs[i] = QC_ENOMEM;
+            }
+            if (rcs[i] != QC_OK) errs[i] = last_error();
+        }
+        (void)hipStreamSynchronize(ts);
+        (void)hipStreamDestroy(ts);
+    };
+    const int nthr = std::max(1, std::min(count, 16));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; t++) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    for (int i = 0; i < count; i++)
+        if (rcs[i] != QC_OK) {
+            set_error("instance %d: %s", i, errs[i].c_str());
+            return rcs[i];
+        }
+    return QC_OK;
+}
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -281,11 +330,8 @@ int qc_mine_batch(const qc_graph* const* graphs, const qc_params* params, int32_
         double t0 = now_ms();
         std::vector<std::unique_ptr<Instance>> insts(count);
         std::vector<const Instance*> ip(count);
-        for (int i = 0; i < count && rc2 == QC_OK; i++) {
-            insts[i].reset(new Instance);
-            rc2 = build_instance(graphs[i], params[i].gamma, params[i].min_size, *insts[i], s);
-            ip[i] = insts[i].get();
-        }
+        if (rc2 == QC_OK) rc2 = build_instances(graphs, params, count, insts);
+        for (int i = 0; i < count; i++) ip[i] = insts[i].get();
         double t_init = now_ms();
         std::vector<EnumOut> eo(count);
         if (rc2 == QC_OK) rc2 = enumerate_persistent(ip.data(), params, count, eo.data(), s);
