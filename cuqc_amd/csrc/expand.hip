@@ -306,8 +306,38 @@ __device__ void sort_keys(Ctx& c, uint64_t* hi, uint64_t* lo, int n) {
 // then P(k) = S[v + 1] + (k - C[v + 1]) v with v the largest value whose
 // C[v] >= k.  Returns V, or -1 when the tables do not fit (the caller sorts).
 // ---------------------------------------------------------------------------
-__device__ int indeg_dist(Ctx& c, int n) {
+__device__ int indeg_dist(Ctx& c, int n, int m) {
     WS& w = c.w;
+    if (m + 2 <= WAVE && m + 2 <= w.T2) {
+        // indegrees lie in [0, m]: lane v counts value v with one ballot per
+        // value and chunk (no atomics), then reversed wave scans give the
+        // suffix tables over the bins m, m-1, ..., 0
+        const int V = m;
+        int cnt = 0;
+        for (int base = 0; base < n; base += WAVE) {
+            const int i = base + c.lane;
+            const int x = i < n ? w.cind[i] : -1;
+            for (int v = 0; v <= V; v++) {
+                const int k = __popcll(ballot(x == v));
+                if (c.lane == v) cnt += k;
+            }
+        }
+        const int bin = V - c.lane;  // lane L holds bin V - L
+        const long long h = c.lane <= V ? (long long)__shfl(cnt, bin < 0 ? 0 : bin, WAVE) : 0;
+        const long long ic = wave_incl_scan64(h);
+        const long long is = wave_incl_scan64(h * (bin > 0 ? bin : 0));
+        wsync();
+        if (c.lane <= V) {
+            w.adjc[bin] = (int)ic;
+            w.cind[bin] = (int)is;
+        }
+        if (c.lane == 0) {
+            w.adjc[V + 1] = 0;
+            w.cind[V + 1] = 0;
+        }
+        wsync();
+        return V;
+    }
     int mx = 0;
     for (int i = c.lane; i < n; i += WAVE) mx = max(mx, w.cind[i]);
     const int V = uni(wave_max(mx));
@@ -469,7 +499,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
                                                               int& med, int nind, bool map_empty) {
     WS& w = c.w;
     ph_mark(c, PH_PRUNE);
-    int V = indeg_dist(c, nind);
+    int V = indeg_dist(c, nind, m);
     if (V < 0) sort_desc_int(c, w.cind, nind);
     if (lu_bounds(c, m, nind, ub, lb, med, V) || members_fail(c, m, ub, lb, med)) {
         ph_mark(c, PH_LU);
@@ -553,7 +583,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
         }
         wsync();
         ph_mark(c, PH_COMPACT);
-        V = indeg_dist(c, nv);
+        V = indeg_dist(c, nv, m);
         if (V < 0) sort_desc_int(c, w.cind, nv);
         if (lu_bounds(c, m, nv, ub, lb, med, V) || members_fail(c, m, ub, lb, med)) {
             ph_mark(c, PH_LU);
