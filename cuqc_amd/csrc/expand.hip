@@ -531,57 +531,57 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             wadj_add(c, wave_sum64(sw));
         }
         // only members and remaining candidates are updated: the removed
-        // ones never return, so their exdeg is never read again
+        // ones never return, so their exdeg is never read again.  The same
+        // pass gathers the indegrees of the remaining candidates that are
+        // valid under the previous bounds with the new exdeg
+        // (main.cu:1811-1815).
         const int nupd = m + nrem;
-#define UPD_IDX(j) ((j) < m ? (j) : w.rem[(j) - m])
-        if (nrem < nrmv && map_empty) {
-            for (int j = c.lane; j < nupd; j += WAVE) w.cex[UPD_IDX(j)] = 0;
-        } else if (!map_empty) {
-            // exdeg = |N(i) n remaining| (recompute) or -= |N(i) n removed|
-            const bool rc = nrem < nrmv;
-            const int nl = rc ? nrem : nrmv;
-            const int* L = rc ? w.rem : w.rmv;
-            if (nl < c.W) {
-                // few vertices: test their bits (nl loads per row, not W)
-                for (int k = c.lane; k < nl; k += WAVE) w.adjc[k] = w.cpp[L[k]];
-                wsync();
-                for (int j = c.lane; j < nupd; j += WAVE) {
-                    const int i = UPD_IDX(j);
-                    const int row = w.cpp[i];
-                    int cnt = 0;
-                    for (int k = 0; k < nl; k++) cnt += (int)gbit(c.M1, c.W, row, w.adjc[k]);
-                    w.cex[i] = rc ? cnt : w.cex[i] - cnt;
-                }
-            } else {
-                clear_mask(c, w.mk0);
-                for (int k = c.lane; k < nl; k += WAVE) set_mask_bit(w.mk0, w.cpp[L[k]]);
-                wsync();
-                for (int j = c.lane; j < nupd; j += WAVE) {
-                    const int i = UPD_IDX(j);
-                    const int cnt = rowpop(c, c.M1, w.cpp[i], w.mk0);
-                    w.cex[i] = rc ? cnt : w.cex[i] - cnt;
-                }
-            }
+        const bool rc = nrem < nrmv;
+        const int nl = rc ? nrem : nrmv;
+        const int* L = rc ? w.rem : w.rmv;
+        // 0: exdeg unchanged, 1: all zero (empty map, recompute),
+        // 2: bit tests against few vertices, 3: row popcounts against a mask
+        const int mode = map_empty ? (rc ? 1 : 0) : (nl < c.W ? 2 : 3);
+        if (mode == 2) {
+            for (int k = c.lane; k < nl; k += WAVE) w.adjc[k] = w.cpp[L[k]];
+            wsync();
+        } else if (mode == 3) {
+            clear_mask(c, w.mk0);
+            for (int k = c.lane; k < nl; k += WAVE) set_mask_bit(w.mk0, w.cpp[L[k]]);
+            wsync();
         }
-#undef UPD_IDX
-        wsync();
-        ph_mark(c, PH_DEGUPD);
-        // indegrees of remaining candidates valid under the previous bounds
         int nv = 0;
-        for (int base = 0; base < nrem; base += WAVE) {
-            int k = base + c.lane;
+        for (int base = 0; base < nupd; base += WAVE) {
+            const int j = base + c.lane;
             bool ok = false;
             int in = 0;
-            if (k < nrem) {
-                int i = w.rem[k];
-                in = w.cin[i];
-                ok = cand_valid(c, in, w.cex[i], m, ub, lb, med);
+            if (j < nupd) {
+                const int i = j < m ? j : w.rem[j - m];
+                int ex = w.cex[i];
+                if (mode == 1) {
+                    ex = 0;
+                } else if (mode >= 2) {
+                    int cnt = 0;
+                    if (mode == 2) {
+                        const int row = w.cpp[i];
+                        for (int k = 0; k < nl; k++) cnt += (int)gbit(c.M1, c.W, row, w.adjc[k]);
+                    } else {
+                        cnt = rowpop(c, c.M1, w.cpp[i], w.mk0);
+                    }
+                    ex = rc ? cnt : ex - cnt;
+                }
+                if (mode != 0) w.cex[i] = ex;
+                if (j >= m) {
+                    in = w.cin[i];
+                    ok = cand_valid(c, in, ex, m, ub, lb, med);
+                }
             }
-            uint64_t bo = ballot(ok);
+            const uint64_t bo = ballot(ok);
             if (ok) w.cind[nv + lanes_below(bo)] = in;
             nv += __popcll(bo);
         }
         wsync();
+        ph_mark(c, PH_DEGUPD);
         ph_mark(c, PH_COMPACT);
         V = indeg_dist(c, nv, m);
         if (V < 0) sort_desc_int(c, w.cind, nv);
@@ -1146,14 +1146,19 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
     const int bbeg = root ? (int)min<long long>(rb0, (long long)nb) : 0;
 
     // remove one vertex, main.cu:1408-1462; returns true on a member failure
-    auto remove_step = [&](bool count) -> bool {
+    // With members_only, candidates keep their exdeg and `pend` names the
+    // removed vertex: the child build applies it to them in its own pass.
+    int pend = -1;
+    auto remove_step = [&](bool count, bool members_only) -> bool {
         int mdg = get_mindeg(c, nm);
         nc--;
         tp--;
         int rr = PM(tp);
+        pend = members_only ? rr : -1;
         bool fail = false;
         int fid = INT32_MAX;  // smallest failing member id
-        for (int i = c.lane; i < tp; i += WAVE) {
+        const int upto = members_only ? nm : tp;
+        for (int i = c.lane; i < upto; i += WAVE) {
             if (gbit(c.M1, c.W, rr, PM(i))) {
                 int ex = w.pex[i] - 1;
                 w.pex[i] = ex;
@@ -1328,10 +1333,28 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         return;
     }
 
+    // A window that starts at branch bbeg > 0 first needs removals
+    // 1..bbeg.  Without members none of them can fail, so they are applied at
+    // once (their adjacency scans counted as the sequential removals count
+    // them) and the loop starts at bbeg.
+    if (!item && bbeg > 0 && nm == 0 && bbeg < bend) {
+        long long sw = 0;
+        for (int k = 1 + c.lane; k <= bbeg; k += WAVE) sw += c.g->d1r[w.pid[t0 - k]];
+        if (count_removes) wadj_add(c, wave_sum64(sw));
+        bulk_remove(t0 - bbeg, bbeg);
+        tp = t0 - bbeg;
+        nc -= bbeg;
+        b_lo = bbeg;
+        ph_mark(c, PH_REMOVE);
+    }
+
     for (int b = b_lo; b < b_hi; b++) {
         if (!item && b >= bend) break;
-        if (b > b_lo || (!item && b > 0)) {
-            const bool stop = remove_step(count_removes);
+        pend = -1;
+        if (b > b_lo || (!item && b > 0 && b_lo == 0)) {
+            // before the window (no child built) the removal is applied
+            // whole; otherwise the child build applies it to the candidates
+            const bool stop = remove_step(count_removes, item || b >= bbeg);
             ph_mark(c, PH_REMOVE);
             if (stop) break;
         }
@@ -1357,11 +1380,13 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             w.cl2[i] = w.pl2[i];
         }
         if (c.lane == 0) {
+            int pv = w.pex[vsrc];
+            if (pend >= 0 && gbit(c.M1, c.W, pend, vpp)) w.pex[vsrc] = --pv;
             w.cid[nm] = w.pid[vsrc];
             w.cpp[nm] = vpp;
             w.clab[nm] = 1;
             w.cin[nm] = w.pin[vsrc];
-            w.cex[nm] = w.pex[vsrc];
+            w.cex[nm] = pv;
             w.cl2[nm] = w.pl2[vsrc];
         }
         int m = nm + 1, cc = nc - 1;
@@ -1384,23 +1409,30 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             const uint64_t bo = smask[k];
             const bool ok = in && ((bo >> c.lane) & 1ull);
             const uint64_t bx = ballot(in && !ok);
+            const int pp = in ? PM(q) : 0;
+            int pexq = 0;
+            if (in) {
+                // the removal of this branch (main.cu:1437-1450) for the
+                // parent's candidate q
+                pexq = w.pex[q];
+                if (pend >= 0 && gbit(c.M1, c.W, pend, pp)) w.pex[q] = --pexq;
+            }
             if (ok) {
                 const int r = lanes_below(bo);
                 const int d = m + nsd + r;
-                const int pp = PM(q);
                 const int a1 = (int)gbit(c.M1, c.W, vpp, pp);
                 const int cin_ = w.pin[q] + a1;
                 w.cid[d] = w.pid[q];
                 w.cpp[d] = pp;
                 w.clab[d] = 0;
                 w.cin[d] = cin_;
-                w.cex[d] = w.pex[q] - a1;
+                w.cex[d] = pexq - a1;
                 w.cl2[d] = w.pl2[q];
                 w.cind[nsd + r] = cin_;
             } else if (in) {
                 const int d = m + ns + ndd + lanes_below(bx);
                 w.cid[d] = w.pid[q];
-                w.cpp[d] = PM(q);
+                w.cpp[d] = pp;
                 w.clab[d] = -1;
             }
             nsd += __popcll(bo);
