@@ -332,7 +332,15 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     if ((rc = I.A2.alloc(mat_bytes))) return rc;
     // LDS form: visited bitmap + both rows of one root vertex per wave
     const size_t lds_bytes = (((size_t)n + 63) / 64 + 2 * (size_t)I.rw) * 8;
-    const bool lds_rows = !g->has_n2 && lds_bytes <= 48 * 1024;
+    int lds_max = 0, cur_dev = 0;
+    QC_HIP(hipGetDevice(&cur_dev));
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, cur_dev) != hipSuccess)
+        lds_max = 64 * 1024;
+    bool lds_rows = !g->has_n2 && lds_bytes <= (size_t)lds_max;
+    if (lds_rows && lds_bytes > 48 * 1024 &&
+        hipFuncSetAttribute((const void*)qcdev::root_rows_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes) != hipSuccess)
+        lds_rows = false;
     if (!lds_rows) {
         QC_HIP(hipMemsetAsync(I.A1.p, 0, mat_bytes, s));
         QC_HIP(hipMemsetAsync(I.A2.p, 0, mat_bytes, s));
