@@ -17,7 +17,9 @@ it):
 * ``workgroups``  resident waves of the persistent launch (0: as many as fit).
 
 The search is coordinate descent from the library defaults, one parameter at
-a time, ``reps`` timed runs per candidate (median kernel time).  Every trial
+a time, repeated while a full sweep improves the time by more than
+``min_gain`` (at most ``max_sweeps`` sweeps), ``reps`` timed runs per
+candidate (median kernel time).  Every trial
 must reproduce the reference schedule's counters (tasks, branches, raw
 cliques, adjacency scans) and raw-set digest: a schedule changes the order of
 work, never the result, and a trial that disagrees raises.
@@ -39,7 +41,7 @@ SPACE: dict[str, list[int]] = {
     "workgroups": [1024, 2048, 3072, 0],
 }
 DEFAULT = {"lds_tile": 128, "split_chunk": 16, "split_min": 0, "mid_tile": 0, "lds_wide": 0, "workgroups": 0}
-ORDER = ("lds_wide", "split_chunk", "split_min", "mid_tile", "lds_tile", "workgroups")
+ORDER = ("split_chunk", "lds_wide", "split_min", "mid_tile", "lds_tile", "workgroups")
 
 _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
 
@@ -76,7 +78,7 @@ def _fingerprint(graphs, gammas, min_sizes, windows) -> str:
 
 
 def tune(graphs, gammas, min_sizes, windows=None, *, reps: int = 2, order: str = "fifo", cost_model=None,
-         cache: str | os.PathLike | None = None, log=None) -> Schedule:
+         cache: str | os.PathLike | None = None, log=None, max_sweeps: int = 3, min_gain: float = 0.01) -> Schedule:
     """Pick the fastest schedule for mining ``graphs`` together (one batched
     persistent launch; a single graph is a batch of one)."""
     import numpy as np
@@ -125,12 +127,20 @@ def tune(graphs, gammas, min_sizes, windows=None, *, reps: int = 2, order: str =
 
     best = dict(DEFAULT)
     best_t = measure(best)
-    for name in ORDER:
-        for v in SPACE[name]:
-            cand = dict(best, **{name: v})
-            t = measure(cand)
-            if t < best_t:
-                best, best_t = cand, t
+    # sweep the parameters one at a time; a parameter tuned early may want
+    # another value once a later one changed, so sweep again while a full
+    # sweep still improves (already measured points are not rerun)
+    for _ in range(max_sweeps):
+        improved = False
+        for name in ORDER:
+            for v in SPACE[name]:
+                cand = dict(best, **{name: v})
+                t = measure(cand)
+                if t < best_t * (1.0 - min_gain):
+                    best, best_t = cand, t
+                    improved = True
+        if not improved:
+            break
     sched = Schedule(kernel_ms=round(best_t, 3), trials=trials, **best)
     if cache is not None:
         known = {}
