@@ -63,6 +63,11 @@ __device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
     return v;
 }
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint64_t)__shfl_xor(v, o, WAVE));
+    return v;
+}
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
@@ -188,9 +193,10 @@ __device__ __forceinline__ void ph_flush(Ctx& c) {
             if (c.ph[k]) atomicAdd(&c.a->phase[k], c.ph[k]);
 }
 
+// mindeg[i] of the host table (graph_io.cpp build_tables), computed in
+// registers: a per-lane table lookup is a dependent memory round trip
 __device__ __forceinline__ int md(const Ctx& c, int i) {
-    // index stays below mindeg_n by construction of the table size
-    return c.g->mindeg[i];
+    return i < 1 ? 0 : (int)ceil(c.g->gamma * (double)(i - 1));
 }
 __device__ __forceinline__ int get_mindeg(const Ctx& c, int s) {
     int ms = c.g->min_size;
@@ -300,44 +306,36 @@ __device__ void sort_keys(Ctx& c, uint64_t* hi, uint64_t* lo, int n) {
 // ---------------------------------------------------------------------------
 // Candidate indegrees for the LU bounds.  The bounds only need P(k), the sum
 // of the k largest indegrees (main.cu:1919-1940 walks them sorted), and
-// indegrees lie in [0, m].  So instead of sorting, the n values in cind are
-// turned into suffix counts C[v] = #{x >= v} (in adjc) and suffix sums
-// S[v] = sum{x : x >= v} (in cind), v in [0, V + 1], V = the largest value;
-// then P(k) = S[v + 1] + (k - C[v + 1]) v with v the largest value whose
-// C[v] >= k.  Returns V, or -1 when the tables do not fit (the caller sorts).
+// indegrees lie in [0, m].  With the suffix counts C[v] = #{x >= v},
+// P(k) = sum over v >= 1 of min(C[v], k): each of the k largest values x
+// adds 1 for every v in [1, x].
+//   m < 64: lane v holds C[v] in a register (creg, one ballot per value and
+//           chunk); reg = true.
+//   else:   tables C[v] (in adjc) and suffix sums S[v] = sum{x : x >= v}
+//           (in cind), v in [0, V + 1]; P(k) = S[v + 1] + (k - C[v + 1]) v
+//           with v the largest value whose C[v] >= k.
+// Returns V = the largest value, or -1 when the tables do not fit (the
+// caller sorts).
 // ---------------------------------------------------------------------------
-__device__ int indeg_dist(Ctx& c, int n, int m) {
+__device__ int indeg_dist(Ctx& c, int n, int m, int& creg, bool& reg) {
     WS& w = c.w;
-    if (m + 2 <= WAVE && m + 2 <= w.T2) {
-        // indegrees lie in [0, m]: lane v counts value v with one ballot per
-        // value and chunk (no atomics), then reversed wave scans give the
-        // suffix tables over the bins m, m-1, ..., 0
-        const int V = m;
-        int cnt = 0;
+    if (m < WAVE) {
+        int cnt = 0, V = 0;
         for (int base = 0; base < n; base += WAVE) {
             const int i = base + c.lane;
             const int x = i < n ? w.cind[i] : -1;
-            for (int v = 0; v <= V; v++) {
-                const int k = __popcll(ballot(x == v));
+            const int top = uni(wave_max(x));
+            V = max(V, top);
+            for (int v = 1; v <= top; v++) {
+                const int k = __popcll(ballot(x >= v));
                 if (c.lane == v) cnt += k;
             }
         }
-        const int bin = V - c.lane;  // lane L holds bin V - L
-        const long long h = c.lane <= V ? (long long)__shfl(cnt, bin < 0 ? 0 : bin, WAVE) : 0;
-        const long long ic = wave_incl_scan64(h);
-        const long long is = wave_incl_scan64(h * (bin > 0 ? bin : 0));
-        wsync();
-        if (c.lane <= V) {
-            w.adjc[bin] = (int)ic;
-            w.cind[bin] = (int)is;
-        }
-        if (c.lane == 0) {
-            w.adjc[V + 1] = 0;
-            w.cind[V + 1] = 0;
-        }
-        wsync();
+        creg = cnt;
+        reg = true;
         return V;
     }
+    reg = false;
     int mx = 0;
     for (int i = c.lane; i < n; i += WAVE) mx = max(mx, w.cind[i]);
     const int V = uni(wave_max(mx));
@@ -372,6 +370,13 @@ __device__ int indeg_dist(Ctx& c, int n, int m) {
     return V;
 }
 
+// P(k) = sum_{v=1..V} min(C[v], k) with lane v's register holding C[v]
+__device__ __forceinline__ int top_sum_reg(int creg, int V, int k) {
+    int p = 0;
+    for (int v = 1; v <= V; v++) p += min(__builtin_amdgcn_readlane(creg, v), k);
+    return p;
+}
+
 // P(k) from the distribution tables (k <= n)
 __device__ __forceinline__ long long top_sum(const WS& w, int V, int k) {
     if (k <= 0) return 0;
@@ -390,27 +395,28 @@ __device__ __forceinline__ long long top_sum(const WS& w, int V, int k) {
 
 // ---------------------------------------------------------------------------
 // LU bounds, main.cu:1857-1972.  Members are child positions [0, m).  With
-// V >= 0 the candidate indegrees are given by the tables of indeg_dist,
-// with V < 0 cind[0..nc) holds them sorted descending.
+// V >= 0 the candidate indegrees are given by indeg_dist (creg / reg), with
+// V < 0 cind[0..nc) holds them sorted descending.
 // ---------------------------------------------------------------------------
-__device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int V) {
+__device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int V, int creg, bool reg) {
     WS& w = c.w;
     const ExpandArgs& a = *c.a;
-    int min_in = INT32_MAX, min_tot = INT32_MAX;
+    // one pass: the smallest (indeg, exdeg) pair in lexicographic order
+    // gives min_in and the smallest exdeg among the members at min_in
+    uint64_t min_pair = ((uint64_t)INT32_MAX << 32) | (uint64_t)INT32_MAX;
+    int min_tot = INT32_MAX;
     long long sum_in = 0;
     for (int i = c.lane; i < m; i += WAVE) {
         int in = w.cin[i], ex = w.cex[i];
-        min_in = min(min_in, in);
+        min_pair = min(min_pair, ((uint64_t)(uint32_t)in << 32) | (uint32_t)ex);
         min_tot = min(min_tot, in + ex);
         sum_in += in;
     }
-    min_in = uni(wave_min(min_in));
+    min_pair = wave_min_u64(min_pair);
+    const int min_in = uni((int)(min_pair >> 32));
+    const int min_in_ex = uni((int)(uint32_t)min_pair);
     min_tot = uni(wave_min(min_tot));
     sum_in = wave_sum64(sum_in);
-    int min_in_ex = INT32_MAX;
-    for (int i = c.lane; i < m; i += WAVE)
-        if (w.cin[i] == min_in) min_in_ex = min(min_in_ex, w.cex[i]);
-    min_in_ex = uni(wave_min(min_in_ex));
 
     bool invalid = false;
     med = get_mindeg(c, m + 1);
@@ -445,7 +451,9 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int
             for (int base = 0; base <= ub; base += WAVE) {
                 int idx = base + c.lane;  // P(idx) = prefix over [0, idx)
                 long long inc;
-                if (V >= 0) {
+                if (V >= 0 && reg) {
+                    inc = top_sum_reg(creg, V, idx);
+                } else if (V >= 0) {
                     inc = idx <= ub ? top_sum(w, V, idx) : 0;
                 } else {
                     long long v = (idx - 1 >= 0 && idx - 1 < nc && idx <= ub) ? (long long)w.cind[idx - 1] : 0;
@@ -499,15 +507,18 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
                                                               int& med, int nind, bool map_empty) {
     WS& w = c.w;
     ph_mark(c, PH_PRUNE);
-    int V = indeg_dist(c, nind, m);
+    int creg = 0;
+    bool reg = false;
+    int V = indeg_dist(c, nind, m, creg, reg);
     if (V < 0) sort_desc_int(c, w.cind, nind);
-    if (lu_bounds(c, m, nind, ub, lb, med, V) || members_fail(c, m, ub, lb, med)) {
+    if (lu_bounds(c, m, nind, ub, lb, med, V, creg, reg) || members_fail(c, m, ub, lb, med)) {
         ph_mark(c, PH_LU);
         return true;
     }
     ph_mark(c, PH_LU);
-    // classify candidates, stable
-    int nrem = 0, nrmv = 0;
+    // classify candidates, stable; first_out = the smallest position ever
+    // removed (positions below it stay in place in the compaction)
+    int nrem = 0, nrmv = 0, first_out = tc;
     for (int base = m; base < tc; base += WAVE) {
         int i = base + c.lane;
         bool in = i < tc;
@@ -515,6 +526,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
         uint64_t bo = ballot(ok), bx = ballot(in && !ok);
         if (ok) w.rem[nrem + lanes_below(bo)] = i;
         if (in && !ok) w.rmv[nrmv + lanes_below(bx)] = i;
+        if (first_out == tc && bx) first_out = base + __builtin_ctzll(bx);
         nrem += __popcll(bo);
         nrmv += __popcll(bx);
     }
@@ -583,9 +595,9 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
         wsync();
         ph_mark(c, PH_DEGUPD);
         ph_mark(c, PH_COMPACT);
-        V = indeg_dist(c, nv, m);
+        V = indeg_dist(c, nv, m, creg, reg);
         if (V < 0) sort_desc_int(c, w.cind, nv);
-        if (lu_bounds(c, m, nv, ub, lb, med, V) || members_fail(c, m, ub, lb, med)) {
+        if (lu_bounds(c, m, nv, ub, lb, med, V, creg, reg) || members_fail(c, m, ub, lb, med)) {
             ph_mark(c, PH_LU);
             return true;
         }
@@ -599,6 +611,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             int i = in ? w.rem[k] : 0;
             bool ok = in && cand_valid(c, w.cin[i], w.cex[i], m, ub, lb, med);
             uint64_t bo = ballot(ok), bx = ballot(in && !ok);
+            if (bx) first_out = min(first_out, __shfl(i, __builtin_ctzll(bx), WAVE));
             wsync();
             if (ok) w.rem[kept + lanes_below(bo)] = i;
             if (in && !ok) w.rmv[nrmv + lanes_below(bx)] = i;
@@ -608,8 +621,10 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
         }
         nrem = kept;
     }
-    // compaction: child[m + k] = child[rem[k]] (rem ascending, m + k <= rem[k])
-    for (int base = 0; base < nrem; base += WAVE) {
+    // compaction: child[m + k] = child[rem[k]] (rem ascending, m + k <= rem[k]);
+    // rem[k] = m + k below first_out - m
+    ph_mark(c, PH_COMPACT);
+    for (int base = first_out - m; base < nrem; base += WAVE) {
         int k = base + c.lane;
         int i = 0, id = 0, pp = 0, lab = 0, in = 0, ex = 0, l2 = 0;
         if (k < nrem) {
@@ -635,7 +650,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
     }
     tc = tc - nc + nrem;
     nc = nrem;
-    ph_mark(c, PH_COMPACT);
+    ph_mark(c, PH_MOVE);
     return false;
 }
 

@@ -96,6 +96,10 @@ struct DevGraph {
     long long root_b0, root_b1;
     int32_t max_depth;
     int32_t pad1;
+    // mindeg[i] = ceil(gamma (i - 1)) is evaluated from this double in the
+    // kernels with the same expression as the host table (one correctly
+    // rounded product, then ceil), so both give the same integers
+    double gamma;
 };
 
 // Arguments of one expansion launch.
@@ -186,7 +190,8 @@ enum PhaseId {
     PH_IDLE,          // persistent pop loop: waiting for work
     PH_LU,            // candidate indegree tables, LU bounds, member checks
     PH_DEGUPD,        // degree-pruning exdeg updates
-    PH_COMPACT,       // degree-pruning classification and compaction
+    PH_COMPACT,       // degree-pruning classification
     PH_CRIT,          // critical-vertex step
+    PH_MOVE,          // degree-pruning compaction of the kept candidates
     PH_COUNT
 };

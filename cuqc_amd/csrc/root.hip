@@ -437,6 +437,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     I.dg.ubq = I.d_ubq.as<int32_t>();
     I.dg.ubq_n = (int32_t)I.ubq.size();
     I.dg.min_size = ms;
+    I.dg.gamma = gamma;
     I.dg.orig = I.d_orig.as<uint32_t>();
     I.dg.d2r = I.d_d2r.as<int32_t>();
     I.dg.d1r = I.d_d1r.as<int32_t>();
