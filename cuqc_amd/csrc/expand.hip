@@ -43,10 +43,54 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Lane broadcasts by v_readlane (lane index uniform): no LDS round trip.
+__device__ __forceinline__ int rdlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ unsigned long long rdlane64(unsigned long long x, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ float rdlanef(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// Whole-wave reductions with DPP moves (called with all 64 lanes active).
+// Quad permutes and the half-row and row mirrors combine every row of 16
+// lanes; row broadcast 15 then adds row 0 into row 1 and row 2 into row 3,
+// row broadcast 31 adds rows 0-1 into rows 2-3; lane 63 holds the result.
+// Lanes a move does not write keep `id`, the operation's identity.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp32(int id, int v) {
+    return __builtin_amdgcn_update_dpp(id, v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ unsigned long long dpp64(unsigned long long id, unsigned long long v) {
+    const uint32_t lo = (uint32_t)dpp32<CTRL, RM>((int)(uint32_t)id, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)dpp32<CTRL, RM>((int)(uint32_t)(id >> 32), (int)(uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+template <class Op>
+__device__ __forceinline__ int wave_reduce32(int v, int id, Op op) {
+    v = op(v, dpp32<0xb1, 0xf>(id, v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp32<0x4e, 0xf>(id, v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp32<0x141, 0xf>(id, v));  // row_half_mirror
+    v = op(v, dpp32<0x140, 0xf>(id, v));  // row_mirror
+    v = op(v, dpp32<0x142, 0xa>(id, v));  // row_bcast:15 into rows 1, 3
+    v = op(v, dpp32<0x143, 0xc>(id, v));  // row_bcast:31 into rows 2, 3
+    return rdlane(v, WAVE - 1);
+}
+template <class Op>
+__device__ __forceinline__ unsigned long long wave_reduce64(unsigned long long v, unsigned long long id, Op op) {
+    v = op(v, dpp64<0xb1, 0xf>(id, v));
+    v = op(v, dpp64<0x4e, 0xf>(id, v));
+    v = op(v, dpp64<0x141, 0xf>(id, v));
+    v = op(v, dpp64<0x140, 0xf>(id, v));
+    v = op(v, dpp64<0x142, 0xa>(id, v));
+    v = op(v, dpp64<0x143, 0xc>(id, v));
+    return rdlane64(v, WAVE - 1);
+}
 __device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
+    return wave_reduce32(v, 0, [](int x, int y) { return x + y; });
 }
 __device__ __forceinline__ float wave_sum_f(float v) {
 #pragma unroll
@@ -54,24 +98,17 @@ __device__ __forceinline__ float wave_sum_f(float v) {
     return v;
 }
 __device__ __forceinline__ long long wave_sum64(long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
+    return (long long)wave_reduce64((unsigned long long)v, 0ull,
+                                    [](unsigned long long x, unsigned long long y) { return x + y; });
 }
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
-    return v;
+    return wave_reduce32(v, INT32_MAX, [](int x, int y) { return min(x, y); });
 }
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint64_t)__shfl_xor(v, o, WAVE));
-    return v;
+    return wave_reduce64(v, ~0ull, [](unsigned long long x, unsigned long long y) { return x < y ? x : y; });
 }
 __device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
-    return v;
+    return wave_reduce32(v, INT32_MIN, [](int x, int y) { return max(x, y); });
 }
 // inclusive prefix sum across the wave
 __device__ __forceinline__ long long wave_incl_scan64(long long v) {
@@ -154,7 +191,7 @@ __device__ inline WS make_ws_split(char* arrays, uint32_t tcap, uint64_t* masks,
 // per-task state
 // ---------------------------------------------------------------------------
 struct LStats {
-    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large, beyond, wadj;
+    unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large, beyond;
 };
 
 struct Ctx {
@@ -172,6 +209,7 @@ struct Ctx {
     long long slot;       // queue slot of the task being expanded (trace)
     unsigned long long* ph;           // phase cycles (LDS, diagnostics)
     long long ph_t;                   // start of the running phase
+    long long wadj_l;                 // this lane's share of W_adj (summed at flush)
 };
 
 // phase accounting: charge the cycles since the last mark to phase k
@@ -233,9 +271,13 @@ __device__ __forceinline__ int rowpop(const Ctx& c, const uint64_t* M, int row, 
 }
 
 // W_adj bookkeeping: adjacency entries the reference's list scans visit
-// (restricted to root vertices), summed over ids
-__device__ __forceinline__ void wadj_add(Ctx& c, long long x) {
-    if (c.lane == 0) c.st->wadj += (unsigned long long)x;
+// (restricted to root vertices), summed over ids.  Each lane keeps its own
+// partial sum; the wave sums them when the counters are flushed.
+__device__ __forceinline__ void wadj_add(Ctx& c, long long x) {  // x uniform
+    if (c.lane == 0) c.wadj_l += x;
+}
+__device__ __forceinline__ void wadj_lanes(Ctx& c, long long x) {  // x per lane
+    c.wadj_l += x;
 }
 
 __device__ inline void clear_mask(Ctx& c, uint64_t* mk) {
@@ -353,8 +395,8 @@ __device__ int indeg_dist(Ctx& c, int n, int m, int& creg, bool& reg) {
         const long long h = v >= 0 ? C[v] : 0;
         const long long ic = wave_incl_scan64(h) + cc;
         const long long is = wave_incl_scan64(h * (v >= 0 ? v : 0)) + cs;
-        cc = __shfl(ic, WAVE - 1, WAVE);
-        cs = __shfl(is, WAVE - 1, WAVE);
+        cc = (long long)rdlane64((unsigned long long)ic, WAVE - 1);
+        cs = (long long)rdlane64((unsigned long long)is, WAVE - 1);
         wsync();
         if (v >= 0) {
             C[v] = (int)ic;
@@ -458,7 +500,7 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int
                 } else {
                     long long v = (idx - 1 >= 0 && idx - 1 < nc && idx <= ub) ? (long long)w.cind[idx - 1] : 0;
                     inc = wave_incl_scan64(v) + carry;  // = P(idx)
-                    carry = __shfl(inc, WAVE - 1, WAVE);
+                    carry = (long long)rdlane64((unsigned long long)inc, WAVE - 1);
                 }
                 bool inr = idx <= ub;
                 bool cnd = inr && (sum_in + inc >= (long long)m * md(c, m + idx));
@@ -540,7 +582,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             const int* L = rc ? w.rem : w.rmv;
             long long sw = 0;
             for (int k = c.lane; k < nl; k += WAVE) sw += c.g->d1r[w.cid[L[k]]];
-            wadj_add(c, wave_sum64(sw));
+            wadj_lanes(c, sw);
         }
         // only members and remaining candidates are updated: the removed
         // ones never return, so their exdeg is never read again.  The same
@@ -611,7 +653,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             int i = in ? w.rem[k] : 0;
             bool ok = in && cand_valid(c, w.cin[i], w.cex[i], m, ub, lb, med);
             uint64_t bo = ballot(ok), bx = ballot(in && !ok);
-            if (bx) first_out = min(first_out, __shfl(i, __builtin_ctzll(bx), WAVE));
+            if (bx) first_out = min(first_out, rdlane(i, __builtin_ctzll(bx)));
             wsync();
             if (ok) w.rem[kept + lanes_below(bo)] = i;
             if (in && !ok) w.rmv[nrmv + lanes_below(bx)] = i;
@@ -679,7 +721,7 @@ __device__ __attribute__((always_inline)) int critical_prep(Ctx& c, int tc, int&
         }
         ncm += __popcll(bc);
     }
-    wadj_add(c, wave_sum64(sw));
+    wadj_lanes(c, sw);
     wsync();
     // critical-adjacent candidates
     int ncrit = 0;
@@ -757,7 +799,7 @@ __device__ __attribute__((always_inline)) int critical_prep(Ctx& c, int tc, int&
         if (!sparse) set_mask_bit(w.mk1, w.cpp[m + k]);
         sw += c.g->d1r[w.cid[m + k]] + c.g->d2r[w.cid[m + k]];
     }
-    wadj_add(c, wave_sum64(sw));
+    wadj_lanes(c, sw);
     wsync();
     for (int i = c.lane; i < tc; i += WAVE) {
         int pp = w.cpp[i];
@@ -813,8 +855,8 @@ __device__ void emit_clique(Ctx& c, const int* ids, int len) {
         ri = atomicAdd(&a.clq_ctr->items, 1ull);
         si = atomicAdd(&a.clq_ctr->slots, (unsigned long long)len);
     }
-    ri = __shfl(ri, 0, WAVE);
-    si = __shfl(si, 0, WAVE);
+    ri = rdlane64(ri, 0);
+    si = rdlane64(si, 0);
     if (ri >= a.clq_ref_cap || si + len > a.clq_vert_cap) {
         if (c.lane == 0) atomicOr(a.error, 1);
         return;
@@ -907,14 +949,14 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
     const int j = c.lane;
     float h1 = bf16f(W[12 * 64 + j]);
     for (int k = 0; k < 12; k++) {
-        float f = __shfl(feat[k], 0, WAVE);
+        float f = rdlanef(feat[k], 0);
         float z = (log1pf(fmaxf(f, 0.0f)) - nrm[k]) / nrm[12 + k];
         h1 = fmaf(z, bf16f(W[k * 64 + j]), h1);
     }
     h1 = fmaxf(h1, 0.0f);
     const int ow2 = 12 * 64 + 64;
     float h2 = bf16f(W[ow2 + 64 * 64 + j]);
-    for (int i = 0; i < 64; i++) h2 = fmaf(__shfl(h1, i, WAVE), bf16f(W[ow2 + i * 64 + j]), h2);
+    for (int i = 0; i < 64; i++) h2 = fmaf(rdlanef(h1, i), bf16f(W[ow2 + i * 64 + j]), h2);
     h2 = fmaxf(h2, 0.0f);
     const int ow3 = ow2 + 64 * 64 + 64;
     float y = wave_sum_f(h2 * bf16f(W[ow3 + j])) + bf16f(W[ow3 + 64]);
@@ -956,8 +998,8 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
             si = atomicAdd(&a.out_ctr[s].slots, (unsigned long long)tc);
         }
     }
-    ri = __shfl(ri, 0, WAVE);
-    si = __shfl(si, 0, WAVE);
+    ri = rdlane64(ri, 0);
+    si = rdlane64(si, 0);
     if (q ? (ri >= a.q_cap || si + tc > a.pool_cap) : (ri >= a.out_ref_cap[s] || si + tc > a.out_vert_cap[s])) {
         if (c.lane == 0) {
             atomicOr(a.error, 1);
@@ -1129,7 +1171,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 set_mask_bit(w.mk0, PM(i));
                 sw += c.g->d2r[w.pid[i]];
             }
-            wadj_add(c, wave_sum64(sw));
+            wadj_lanes(c, sw);
             wsync();
             bool bad = false;
             for (int i = nm + c.lane; i < t0; i += WAVE) {
@@ -1197,7 +1239,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                     if (wd == (fid >> 6)) x &= (1ull << (fid & 63)) - 1ull;
                     rank += __popcll(x);
                 }
-                wadj_add(c, (long long)wave_sum(rank) + 1);
+                wadj_lanes(c, rank + (c.lane == 0));
             }
             return true;
         }
@@ -1279,7 +1321,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // the partial scan up to the smallest failing member for removal kb
         long long sw = 0;
         for (int k = 1 + c.lane; k < kb; k += WAVE) sw += c.g->d1r[w.pid[t0 - k]];
-        wadj_add(c, wave_sum64(sw));
+        wadj_lanes(c, sw);
         if (kb < bend) {
             int fid = INT32_MAX;
             for (int i = c.lane; i < nm; i += WAVE)
@@ -1292,7 +1334,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 if (wd == (fid >> 6)) x &= (1ull << (fid & 63)) - 1ull;
                 rank += __popcll(x);
             }
-            wadj_add(c, (long long)wave_sum(rank) + 1);
+            wadj_lanes(c, rank + (c.lane == 0));
         }
         const int hi = min(kb, bend);
         const int ch = a.split_chunk;
@@ -1306,7 +1348,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         bulk_remove(ts, pre);
         unsigned long long slot = 0;
         if (c.lane == 0) slot = atomicAdd(a.pool_top, (unsigned long long)ts);
-        slot = __shfl(slot, 0, WAVE);
+        slot = rdlane64(slot, 0);
         if (slot + ts > a.pool_cap) {
             if (c.lane == 0) atomicOr(a.error, 1);
             return;
@@ -1355,7 +1397,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
     if (!item && bbeg > 0 && nm == 0 && bbeg < bend) {
         long long sw = 0;
         for (int k = 1 + c.lane; k <= bbeg; k += WAVE) sw += c.g->d1r[w.pid[t0 - k]];
-        if (count_removes) wadj_add(c, wave_sum64(sw));
+        if (count_removes) wadj_lanes(c, sw);
         bulk_remove(t0 - bbeg, bbeg);
         tp = t0 - bbeg;
         nc -= bbeg;
@@ -1499,6 +1541,8 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
 // kernels
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void flush_stats(Ctx& c, DevStats* d) {
+    const unsigned long long wadj = (unsigned long long)wave_sum64(c.wadj_l);
+    c.wadj_l = 0;
     if (c.lane == 0) {
         const LStats& t = *c.st;
         if (t.tasks) atomicAdd(&d->tasks, t.tasks);
@@ -1510,8 +1554,8 @@ __device__ __forceinline__ void flush_stats(Ctx& c, DevStats* d) {
         if (t.sum_clq) atomicAdd(&d->sum_clique_len, t.sum_clq);
         if (t.large) atomicAdd(&d->large_tasks, t.large);
         if (t.beyond) atomicAdd(&d->beyond, t.beyond);
-        if (t.wadj) atomicAdd(&d->w_adj, t.wadj);
-        *c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (wadj) atomicAdd(&d->w_adj, wadj);
+        *c.st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     }
 }
 __device__ __forceinline__ void flush_stats(Ctx& c) { flush_stats(c, c.a->stats); }
@@ -1526,7 +1570,7 @@ __device__ __forceinline__ void task_loop(Ctx& c) {
     } else {
         uint32_t x = 0;
         if (c.lane == 0) x = atomicAdd(a.task_counter, 1u);
-        i = uni((int)__shfl(x, 0, WAVE));
+        i = (uint32_t)rdlane((int)x, 0);
     }
     while (i < a.n_in) {
         TaskRef ref = a.in_refs[i];
@@ -1540,7 +1584,7 @@ __device__ __forceinline__ void task_loop(Ctx& c) {
         } else {
             uint32_t x = 0;
             if (c.lane == 0) x = atomicAdd(a.task_counter, 1u);
-            i = uni((int)__shfl(x, 0, WAVE));
+            i = (uint32_t)rdlane((int)x, 0);
         }
     }
     flush_stats(c);
@@ -1552,6 +1596,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     constexpr size_t BYTES = ws_bytes(TL, true, (TL + 63) / 64);
     __shared__ __attribute__((aligned(16))) char lds[BYTES];
     Ctx c;
+    c.wadj_l = 0;
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
@@ -1563,7 +1608,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     __shared__ unsigned long long s_ph[PH_COUNT];
     c.st = &s_st;
     c.ph = s_ph;
-    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     task_loop<true>(c);
 }
@@ -1571,6 +1616,7 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
 __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ ap) {
     const ExpandArgs& a = *ap;
     Ctx c;
+    c.wadj_l = 0;
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
@@ -1583,7 +1629,7 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     __shared__ unsigned long long s_ph[PH_COUNT];
     c.st = &s_st;
     c.ph = s_ph;
-    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     task_loop<false>(c);
 }
@@ -1599,6 +1645,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     // dynamic LDS: max(TL workspace with local matrices, arrays of wide tasks)
     extern __shared__ __attribute__((aligned(16))) char lds[];
     Ctx c;
+    c.wadj_l = 0;
     c.a = ap;
     c.g = &ap->g;
     c.inst = 0;
@@ -1611,7 +1658,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     __shared__ unsigned long long s_ph[PH_COUNT];
     c.st = &s_st;
     c.ph = s_ph;
-    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
     constexpr int WAIT_LIMIT = 1 << 22;
     int idle = 0;      // consecutive empty polls (back-off)
@@ -1645,7 +1692,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                     state = 2;
             }
         }
-        state = uni(__shfl(state, 0, WAVE));
+        state = rdlane(state, 0);
         if (state == 2) break;
         if (state == 3) continue;
         if (state == 0) {
@@ -1654,7 +1701,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
             continue;
         }
         idle = 0;
-        h = __shfl(h, 0, WAVE);
+        h = rdlane64(h, 0);
         unsigned int rd = 0;
         if (c.lane == 0) {
             // the producer published or is about to publish this slot
@@ -1665,7 +1712,7 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
             }
             if (!rd) atomicOr(a.error, 4);
         }
-        rd = (unsigned int)uni((int)__shfl(rd, 0, WAVE));
+        rd = (unsigned int)rdlane((int)rd, 0);
         ph_mark(c, PH_IDLE);
         if (rd == 1) {
             const TaskRef ref = a.q_refs[h];
