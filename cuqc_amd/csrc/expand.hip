@@ -194,9 +194,37 @@ struct LStats {
     unsigned long long tasks, branches, children, raw, sum_t, sum_tw, sum_clq, large, beyond;
 };
 
+// The scalar fields of a DevGraph the kernels read, held in scalar
+// registers for the task being expanded: read through the table pointer they
+// would be reloaded from memory after every store the compiler cannot
+// disambiguate.  The table pointers stay behind `p` (holding all of them in
+// scalar registers spills other scalars into vector lanes).
+struct GConst {
+    const DevGraph* p;
+    double gamma;
+    int32_t ubq_n, min_size, max_depth;
+};
+
+__device__ __forceinline__ int32_t uni32(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long long uni64(long long x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((unsigned long long)x >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ GConst load_graph(const DevGraph* p) {
+    GConst g;
+    g.p = p;
+    g.gamma = __longlong_as_double(uni64(__double_as_longlong(p->gamma)));
+    g.ubq_n = uni32(p->ubq_n);
+    g.min_size = uni32(p->min_size);
+    g.max_depth = uni32(p->max_depth);
+    return g;
+}
+
 struct Ctx {
     const ExpandArgs* a;
-    const DevGraph* g;    // tables of the instance being expanded
+    GConst g;             // tables of the instance being expanded
     WS w;
     const uint64_t* M1;   // matrix used for tests (local or root space)
     const uint64_t* M2;
@@ -210,11 +238,12 @@ struct Ctx {
     unsigned long long* ph;           // phase cycles (LDS, diagnostics)
     long long ph_t;                   // start of the running phase
     long long wadj_l;                 // this lane's share of W_adj (summed at flush)
+    bool phase_on;                    // phase counters requested
 };
 
 // phase accounting: charge the cycles since the last mark to phase k
 __device__ __forceinline__ void ph_mark(Ctx& c, int k) {
-    if (c.a->phase) {
+    if (c.phase_on) {
         long long n = clock64();
         if (c.lane == 0) c.ph[k] += (unsigned long long)(n - c.ph_t);
         c.ph_t = n;
@@ -234,17 +263,17 @@ __device__ __forceinline__ void ph_flush(Ctx& c) {
 // mindeg[i] of the host table (graph_io.cpp build_tables), computed in
 // registers: a per-lane table lookup is a dependent memory round trip
 __device__ __forceinline__ int md(const Ctx& c, int i) {
-    return i < 1 ? 0 : (int)ceil(c.g->gamma * (double)(i - 1));
+    return i < 1 ? 0 : (int)ceil(c.g.gamma * (double)(i - 1));
 }
 __device__ __forceinline__ int get_mindeg(const Ctx& c, int s) {
-    int ms = c.g->min_size;
+    int ms = c.g.min_size;
     return s < ms ? md(c, ms) : md(c, s);
 }
 
 // main.cu:2153-2173
 __device__ __forceinline__ bool cand_valid(const Ctx& c, int in, int ex, int m, int ub, int lb, int med) {
     int tot = in + ex;
-    if (tot < md(c, c.g->min_size)) return false;
+    if (tot < md(c, c.g.min_size)) return false;
     if (tot < get_mindeg(c, m + ex + 1)) return false;
     if (tot < med) return false;
     if (in + ub - 1 < md(c, m + lb)) return false;
@@ -254,7 +283,7 @@ __device__ __forceinline__ bool cand_valid(const Ctx& c, int in, int ex, int m, 
 // main.cu:2175-2198
 __device__ __forceinline__ bool vert_ext(const Ctx& c, int in, int ex, int m, int ub, int lb, int med) {
     int tot = in + ex;
-    if (tot < md(c, c.g->min_size)) return false;
+    if (tot < md(c, c.g.min_size)) return false;
     if (tot < get_mindeg(c, m + ex)) return false;
     if (tot < med) return false;
     if (ex == 0 && in < get_mindeg(c, m + ex)) return false;
@@ -268,6 +297,20 @@ __device__ __forceinline__ int rowpop(const Ctx& c, const uint64_t* M, int row, 
     int s = 0;
     for (int w = 0; w < c.W; w++) s += __popcll(r[w] & mask[w]);
     return s;
+}
+// rowpop with four independent loads per step (the gather latency
+// overlapped), for the loops that run it over whole records
+__device__ __forceinline__ int rowpop4(const Ctx& c, const uint64_t* M, int row, const uint64_t* mask) {
+    const uint64_t* r = M + (size_t)row * c.W;
+    const int W = c.W;
+    int s0 = 0, s1 = 0, w = 0;
+    for (; w + 4 <= W; w += 4) {
+        const uint64_t x0 = r[w], x1 = r[w + 1], x2 = r[w + 2], x3 = r[w + 3];
+        s0 += __popcll(x0 & mask[w]) + __popcll(x2 & mask[w + 2]);
+        s1 += __popcll(x1 & mask[w + 1]) + __popcll(x3 & mask[w + 3]);
+    }
+    for (; w < W; w++) s0 += __popcll(r[w] & mask[w]);
+    return s0 + s1;
 }
 
 // W_adj bookkeeping: adjacency entries the reference's list scans visit
@@ -481,7 +524,7 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int
             lb = nc + 1;
             invalid = true;
         }
-        int tq = min_tot < c.g->ubq_n ? c.g->ubq[min_tot] : c.g->ubq[c.g->ubq_n - 1];
+        int tq = min_tot < c.g.ubq_n ? c.g.p->ubq[min_tot] : c.g.p->ubq[c.g.ubq_n - 1];
         ub = tq + 1 - m;
         if (ub > nc) ub = nc;
         if (lb < ub) {
@@ -525,10 +568,10 @@ __device__ bool lu_bounds(Ctx& c, int m, int nc, int& ub, int& lb, int& med, int
         }
     } else {
         ub = nc;
-        int ms = c.g->min_size;
+        int ms = c.g.min_size;
         lb = m < ms ? ms - m : 0;
     }
-    if (m + ub < c.g->min_size) invalid = true;
+    if (m + ub < c.g.min_size) invalid = true;
     if (ub < 0 || ub < lb) invalid = true;
     return invalid;
 }
@@ -581,7 +624,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             const int nl = rc ? nrem : nrmv;
             const int* L = rc ? w.rem : w.rmv;
             long long sw = 0;
-            for (int k = c.lane; k < nl; k += WAVE) sw += c.g->d1r[w.cid[L[k]]];
+            for (int k = c.lane; k < nl; k += WAVE) sw += c.g.p->d1r[w.cid[L[k]]];
             wadj_lanes(c, sw);
         }
         // only members and remaining candidates are updated: the removed
@@ -717,7 +760,7 @@ __device__ __attribute__((always_inline)) int critical_prep(Ctx& c, int tc, int&
         const uint64_t bc = ballot(cr);
         if (cr) {
             w.rmv[ncm + lanes_below(bc)] = w.cpp[k];
-            sw += c.g->d1r[w.cid[k]];
+            sw += c.g.p->d1r[w.cid[k]];
         }
         ncm += __popcll(bc);
     }
@@ -797,7 +840,7 @@ __device__ __attribute__((always_inline)) int critical_prep(Ctx& c, int tc, int&
     sw = 0;
     for (int k = c.lane; k < ncrit; k += WAVE) {
         if (!sparse) set_mask_bit(w.mk1, w.cpp[m + k]);
-        sw += c.g->d1r[w.cid[m + k]] + c.g->d2r[w.cid[m + k]];
+        sw += c.g.p->d1r[w.cid[m + k]] + c.g.p->d2r[w.cid[m + k]];
     }
     wadj_lanes(c, sw);
     wsync();
@@ -861,7 +904,7 @@ __device__ void emit_clique(Ctx& c, const int* ids, int len) {
         if (c.lane == 0) atomicOr(a.error, 1);
         return;
     }
-    for (int k = c.lane; k < len; k += WAVE) a.clq_verts[si + k] = c.g->orig[ids[k]];
+    for (int k = c.lane; k < len; k += WAVE) a.clq_verts[si + k] = c.g.p->orig[ids[k]];
     if (c.lane == 0) {
         CliqueRef r;
         r.off = si;
@@ -911,7 +954,7 @@ __device__ void task_features(Ctx& c, int tc, int mem, int depth, int gap, float
         mx = max(mx, ex);
         si += in;
         se += ex;
-        if (i >= mem) s2 += c.g->d2r[w.cid[i]];
+        if (i >= mem) s2 += c.g.p->d2r[w.cid[i]];
     }
     mi = wave_min(mi);
     ma = wave_max(ma);
@@ -969,7 +1012,7 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
 __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int mem, int gap) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
-    if (c.g->max_depth >= 0 && (int)c.depth + 1 > c.g->max_depth) {
+    if (c.g.max_depth >= 0 && (int)c.depth + 1 > c.g.max_depth) {
         // outside the depth window: counted, never expanded
         if (c.lane == 0) {
             c.st->children++;
@@ -1025,7 +1068,7 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     // slot bound of this child's own children (see TaskRef)
     unsigned long long bnd = 0;
     for (int i = mem + c.lane; i < tc; i += WAVE) {
-        unsigned long long x = (unsigned long long)c.g->d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
+        unsigned long long x = (unsigned long long)c.g.p->d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
         bnd += x < (unsigned long long)tc ? x : (unsigned long long)tc;
     }
     bnd = (unsigned long long)wave_sum64((long long)bnd);
@@ -1050,7 +1093,8 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     if (q && a.trace_feat && c.lane == 0)
         for (int k = 0; k < 12; k++) a.trace_feat[ri * 12 + k] = feat[k];
     if (q) {
-        __threadfence();
+        // the release store publishes the whole wave's record and ref writes
+        wsync();
         if (c.lane == 0) __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (c.lane == 0) {
@@ -1108,7 +1152,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         c.W = W;
         c.M1 = w.M1;
         c.M2 = w.M2;
-        const int rw = c.g->rw;
+        const int rw = c.g.p->rw;
         for (int wd = 0; wd < W; wd++) {
             int col = wd * 64 + c.lane;
             bool cv = col < t0;
@@ -1118,10 +1162,10 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             int i = 0;
             for (; i + 4 <= t0; i += 4) {
                 int r0 = w.pid[i], r1 = w.pid[i + 1], r2 = w.pid[i + 2], r3 = w.pid[i + 3];
-                uint64_t x0 = c.g->A1[(size_t)r0 * rw + cw], x1 = c.g->A1[(size_t)r1 * rw + cw];
-                uint64_t x2 = c.g->A1[(size_t)r2 * rw + cw], x3 = c.g->A1[(size_t)r3 * rw + cw];
-                uint64_t y0 = c.g->A2[(size_t)r0 * rw + cw], y1 = c.g->A2[(size_t)r1 * rw + cw];
-                uint64_t y2 = c.g->A2[(size_t)r2 * rw + cw], y3 = c.g->A2[(size_t)r3 * rw + cw];
+                uint64_t x0 = c.g.p->A1[(size_t)r0 * rw + cw], x1 = c.g.p->A1[(size_t)r1 * rw + cw];
+                uint64_t x2 = c.g.p->A1[(size_t)r2 * rw + cw], x3 = c.g.p->A1[(size_t)r3 * rw + cw];
+                uint64_t y0 = c.g.p->A2[(size_t)r0 * rw + cw], y1 = c.g.p->A2[(size_t)r1 * rw + cw];
+                uint64_t y2 = c.g.p->A2[(size_t)r2 * rw + cw], y3 = c.g.p->A2[(size_t)r3 * rw + cw];
                 uint64_t b0 = ballot(cv && (x0 & cm)), b1 = ballot(cv && (x1 & cm));
                 uint64_t b2 = ballot(cv && (x2 & cm)), b3 = ballot(cv && (x3 & cm));
                 uint64_t e0 = ballot(cv && (y0 & cm)), e1 = ballot(cv && (y1 & cm));
@@ -1139,8 +1183,8 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             }
             for (; i < t0; i++) {
                 int r0 = w.pid[i];
-                uint64_t x0 = c.g->A1[(size_t)r0 * rw + cw];
-                uint64_t y0 = c.g->A2[(size_t)r0 * rw + cw];
+                uint64_t x0 = c.g.p->A1[(size_t)r0 * rw + cw];
+                uint64_t y0 = c.g.p->A2[(size_t)r0 * rw + cw];
                 uint64_t b0 = ballot(cv && (x0 & cm)), e0 = ballot(cv && (y0 & cm));
                 if (c.lane == 0) {
                     w.M1[(size_t)i * W + wd] = b0;
@@ -1150,9 +1194,9 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         }
         wsync();
     } else {
-        c.W = c.g->rw;
-        c.M1 = c.g->A1;
-        c.M2 = c.g->A2;
+        c.W = c.g.p->rw;
+        c.M1 = c.g.p->A1;
+        c.M2 = c.g.p->A2;
     }
 #define PM(i) (LOCAL ? (i) : P2M[(i)])
     ph_mark(c, PH_LOAD);
@@ -1165,17 +1209,21 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         for (int i = c.lane; i < nm; i += WAVE)
             if (w.pin[i] + w.pex[i] < req) f = true;
         if (ballot(f) == 0) {
-            clear_mask(c, w.mk0);
+            // a record holding every root vertex and no member is the whole
+            // root: |N2(v) n candidates| is then |N2(v) n root| = d2r[v],
+            // with no row scans (O(R^2 / 64) words at the root otherwise)
+            const bool whole_root = nm == 0 && t0 == c.g.p->R;
+            if (!whole_root) clear_mask(c, w.mk0);
             long long sw = 0;
             for (int i = nm + c.lane; i < t0; i += WAVE) {
-                set_mask_bit(w.mk0, PM(i));
-                sw += c.g->d2r[w.pid[i]];
+                if (!whole_root) set_mask_bit(w.mk0, PM(i));
+                sw += c.g.p->d2r[w.pid[i]];
             }
             wadj_lanes(c, sw);
             wsync();
             bool bad = false;
             for (int i = nm + c.lane; i < t0; i += WAVE) {
-                int l2 = rowpop(c, c.M2, PM(i), w.mk0);
+                int l2 = whole_root ? c.g.p->d2r[w.pid[i]] : rowpop(c, c.M2, PM(i), w.mk0);
                 w.pl2[i] = l2;
                 if (l2 < nc - 1 || w.pin[i] + w.pex[i] < req) bad = true;
             }
@@ -1197,9 +1245,9 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
     const bool root = ref.depth == 0;
     const int nb = expansions - cov;
     int bend = nb;
-    if (root && c.g->root_b1 >= 0 && c.g->root_b1 < bend) bend = (int)c.g->root_b1;
+    if (root && c.g.p->root_b1 >= 0 && c.g.p->root_b1 < bend) bend = (int)c.g.p->root_b1;
     // a negative window start counts from the end of the root's branches
-    const long long rb0 = c.g->root_b0 >= 0 ? c.g->root_b0 : max(0ll, (long long)nb + c.g->root_b0);
+    const long long rb0 = c.g.p->root_b0 >= 0 ? c.g.p->root_b0 : max(0ll, (long long)nb + c.g.p->root_b0);
     const int bbeg = root ? (int)min<long long>(rb0, (long long)nb) : 0;
 
     // remove one vertex, main.cu:1408-1462; returns true on a member failure
@@ -1232,7 +1280,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 // after the first failing member
                 fid = uni(wave_min(fid));
                 const int rid = w.pid[tp];
-                const uint64_t* row = c.g->A1 + (size_t)rid * c.g->rw;
+                const uint64_t* row = c.g.p->A1 + (size_t)rid * c.g.p->rw;
                 int rank = 0;
                 for (int wd = c.lane; wd <= (fid >> 6); wd += WAVE) {
                     uint64_t x = row[wd];
@@ -1243,7 +1291,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             }
             return true;
         }
-        if (count) wadj_add(c, c.g->d1r[w.pid[tp]]);
+        if (count) wadj_add(c, c.g.p->d1r[w.pid[tp]]);
         return false;
     };
 
@@ -1264,7 +1312,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             clear_mask(c, w.mk0);
             for (int q = keep + c.lane; q < keep + nr; q += WAVE) set_mask_bit(w.mk0, PM(q));
             wsync();
-            for (int i = c.lane; i < keep; i += WAVE) w.pex[i] -= rowpop(c, c.M1, PM(i), w.mk0);
+            for (int i = c.lane; i < keep; i += WAVE) w.pex[i] -= rowpop4(c, c.M1, PM(i), w.mk0);
         }
         wsync();
     };
@@ -1320,14 +1368,14 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // W_adj of the removals: full scans of N(r) for removals 1..kb-1,
         // the partial scan up to the smallest failing member for removal kb
         long long sw = 0;
-        for (int k = 1 + c.lane; k < kb; k += WAVE) sw += c.g->d1r[w.pid[t0 - k]];
+        for (int k = 1 + c.lane; k < kb; k += WAVE) sw += c.g.p->d1r[w.pid[t0 - k]];
         wadj_lanes(c, sw);
         if (kb < bend) {
             int fid = INT32_MAX;
             for (int i = c.lane; i < nm; i += WAVE)
                 if (w.rem[i] == kb) fid = min(fid, w.pid[i]);
             fid = uni(wave_min(fid));
-            const uint64_t* rrow = c.g->A1 + (size_t)w.pid[t0 - kb] * c.g->rw;
+            const uint64_t* rrow = c.g.p->A1 + (size_t)w.pid[t0 - kb] * c.g.p->rw;
             int rank = 0;
             for (int wd = c.lane; wd <= (fid >> 6); wd += WAVE) {
                 uint64_t x = rrow[wd];
@@ -1361,30 +1409,43 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             v.exdeg = w.pex[i];
             a.pool[slot + i] = v;
         }
-        __threadfence();
-        for (int b0 = bbeg; b0 < hi; b0 += ch) {
-            if (c.lane == 0) {
-                const int top = a.n_buckets - 1;
-                unsigned long long ri = atomicAdd(&a.q_tail[top * QSTRIDE], 1ull);
-                if (ri >= a.q_cap) {
-                    atomicOr(a.error, 1);
-                } else {
-                    ri += (unsigned long long)top * a.q_cap;
-                    TaskRef r;
-                    r.off = slot;
-                    r.t = (uint32_t)ts;
-                    r.depth = ref.depth;
-                    r.inst = ref.inst;
-                    r.bound = (uint64_t)c.slot;  // the task the item belongs to
-                    if (a.trace_parent) a.trace_parent[ri] = -2;
-                    r.b0 = (uint32_t)(b0 - pre);
-                    r.b1 = (uint32_t)(min(b0 + ch, hi) - pre);
-                    atomicAdd((unsigned long long*)a.pending, 1ull);
-                    a.q_refs[ri] = r;
-                    __threadfence();
-                    __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
+        // the items: all slots claimed at once, refs written by the lanes,
+        // one release fence, then the ready flags (relaxed stores)
+        const int n_items = (hi - bbeg + ch - 1) / ch;
+        const int top = a.n_buckets - 1;
+        unsigned long long ri0 = 0;
+        if (c.lane == 0) {
+            ri0 = atomicAdd(&a.q_tail[top * QSTRIDE], (unsigned long long)n_items);
+            // only slots inside the queue count as pending (the rest are
+            // an overflow: the launch is rerun with larger queues)
+            const long long n_ok = ri0 >= a.q_cap ? 0 : (long long)min<unsigned long long>(n_items, a.q_cap - ri0);
+            if (n_ok > 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)n_ok);
+            if (n_ok < n_items) atomicOr(a.error, 1);
+        }
+        ri0 = rdlane64(ri0, 0);
+        for (int j = c.lane; j < n_items; j += WAVE) {
+            const unsigned long long ri = ri0 + (unsigned long long)j;
+            if (ri >= a.q_cap) continue;
+            const unsigned long long gi = ri + (unsigned long long)top * a.q_cap;
+            const int b0 = bbeg + j * ch;
+            TaskRef r;
+            r.off = slot;
+            r.t = (uint32_t)ts;
+            r.depth = ref.depth;
+            r.inst = ref.inst;
+            r.bound = (uint64_t)c.slot;  // the task the item belongs to
+            r.b0 = (uint32_t)(b0 - pre);
+            r.b1 = (uint32_t)(min(b0 + ch, hi) - pre);
+            if (a.trace_parent) a.trace_parent[gi] = -2;
+            a.q_refs[gi] = r;
+        }
+        wsync();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        for (int j = c.lane; j < n_items; j += WAVE) {
+            const unsigned long long ri = ri0 + (unsigned long long)j;
+            if (ri < a.q_cap)
+                __hip_atomic_store(&a.q_ready[ri + (unsigned long long)top * a.q_cap], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         ph_mark(c, PH_PREPARE);
         return;
@@ -1396,7 +1457,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
     // them) and the loop starts at bbeg.
     if (!item && bbeg > 0 && nm == 0 && bbeg < bend) {
         long long sw = 0;
-        for (int k = 1 + c.lane; k <= bbeg; k += WAVE) sw += c.g->d1r[w.pid[t0 - k]];
+        for (int k = 1 + c.lane; k <= bbeg; k += WAVE) sw += c.g.p->d1r[w.pid[t0 - k]];
         if (count_removes) wadj_lanes(c, sw);
         bulk_remove(t0 - bbeg, bbeg);
         tp = t0 - bbeg;
@@ -1425,7 +1486,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         if (c.lane == 0) c.st->branches++;
         const int vsrc = tp - 1;
         const int vpp = PM(vsrc);
-        wadj_add(c, (long long)c.g->d1r[w.pid[vsrc]] + c.g->d2r[w.pid[vsrc]]);
+        wadj_add(c, (long long)c.g.p->d1r[w.pid[vsrc]] + c.g.p->d2r[w.pid[vsrc]]);
         for (int i = c.lane; i < nm; i += WAVE) {
             const int pp = PM(i);
             const int a1 = (int)gbit(c.M1, c.W, vpp, pp);
@@ -1511,7 +1572,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 break;
             }
             if (fail) {
-                if (m >= c.g->min_size) check_for_clique(c, m);
+                if (m >= c.g.min_size) check_for_clique(c, m);
                 skip = true;
                 break;
             }
@@ -1527,7 +1588,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             ph_mark(c, PH_PRUNE);
             continue;
         }
-        if (m >= c.g->min_size) check_for_clique(c, m);
+        if (m >= c.g.min_size) check_for_clique(c, m);
         ph_mark(c, PH_PRUNE);
         if (r == 1) continue;
         if (cc > 0) write_child(c, tc, m, ub - lb);
@@ -1598,7 +1659,8 @@ __global__ __launch_bounds__(64) void expand_local(const ExpandArgs* __restrict_
     Ctx c;
     c.wadj_l = 0;
     c.a = ap;
-    c.g = &ap->g;
+    c.g = load_graph(&ap->g);
+    c.phase_on = ap->phase != nullptr;
     c.inst = 0;
     c.slot = -1;
     c.lane = lane_id();
@@ -1618,7 +1680,8 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
     Ctx c;
     c.wadj_l = 0;
     c.a = ap;
-    c.g = &ap->g;
+    c.g = load_graph(&ap->g);
+    c.phase_on = ap->phase != nullptr;
     c.inst = 0;
     c.slot = -1;
     c.lane = lane_id();
@@ -1647,7 +1710,8 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     Ctx c;
     c.wadj_l = 0;
     c.a = ap;
-    c.g = &ap->g;
+    c.g = load_graph(&ap->g);
+    c.phase_on = ap->phase != nullptr;
     c.inst = 0;
     c.slot = -1;
     c.lane = lane_id();
@@ -1664,8 +1728,9 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
     int idle = 0;      // consecutive empty polls (back-off)
     int sinst = -1;    // instance the wave's counters belong to
     for (;;) {
-        int state = 0;  // 0 nothing published, 1 got a slot, 2 done, 3 lost a race
+        int state = 0;  // 0 nothing published, 1 got slots, 2 done, 3 lost a race
         unsigned long long h = 0;
+        int nk = 1;     // consecutive slots claimed
         if (c.lane == 0) {
             int err = __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (err & 4) {
@@ -1678,17 +1743,26 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
                     unsigned long long tb =
                         __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (hb < tb && hb < a.q_cap) {
-                        if (atomicCAS(hp, hb, hb + 1) == hb) {
+                        // a deep queue hands out several consecutive slots
+                        // per claim: one head update per claim limits how
+                        // fast tiny tasks can be taken
+                        const unsigned long long avail = min(tb, a.q_cap) - hb;
+                        const unsigned long long grid = gridDim.x;
+                        const int k = avail >= 8 * grid ? 4 : avail >= 4 * grid ? 2 : 1;
+                        if (atomicCAS(hp, hb, hb + k) == hb) {
                             state = 1;
                             h = hb + (unsigned long long)b * a.q_cap;
+                            nk = k;
                         } else {
                             state = 3;
                         }
                     }
                 }
                 // every queue looked empty: finished once nothing is pending
-                // (a task counts itself out only after pushing its children)
-                if (state == 0 && __hip_atomic_load(a.pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+                // (a task counts itself out only after pushing its children).
+                // Polls are relaxed: an acquire invalidates the CU's vector
+                // L1 under the waves still working.
+                if (state == 0 && __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
                     state = 2;
             }
         }
@@ -1702,59 +1776,87 @@ __global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __rest
         }
         idle = 0;
         h = rdlane64(h, 0);
-        unsigned int rd = 0;
-        if (c.lane == 0) {
-            // the producer published or is about to publish this slot
-            for (int it = 0; it < WAIT_LIMIT; it++) {
-                rd = __hip_atomic_load(&a.q_ready[h], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if (rd) break;
-                __builtin_amdgcn_s_sleep(2);
+        nk = rdlane(nk, 0);
+        for (int q = 0; q < nk; q++) {
+            const unsigned long long hq = h + (unsigned long long)q;
+            unsigned int rd = 0;
+            if (c.lane == 0) {
+                // the producer published or is about to publish this slot
+                for (int it = 0; it < WAIT_LIMIT; it++) {
+                    rd = __hip_atomic_load(&a.q_ready[hq], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (rd) break;
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                if (!rd) atomicOr(a.error, 4);
             }
-            if (!rd) atomicOr(a.error, 4);
-        }
-        rd = (unsigned int)rdlane((int)rd, 0);
-        ph_mark(c, PH_IDLE);
-        if (rd == 1) {
-            const TaskRef ref = a.q_refs[h];
-            if (a.graphs) {
-                c.g = &a.graphs[ref.inst];
-                c.inst = ref.inst;
+            rd = (unsigned int)rdlane((int)rd, 0);
+            ph_mark(c, PH_IDLE);
+            if (rd == 1) {
+                // one acquire per task taken (pairs with the producer's release
+                // store of the ready flag)
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const TaskRef ref = a.q_refs[hq];
+                const long long tl_start = a.timeline ? wall_clock64() : 0;
+                if (a.graphs) {
+                    c.g = load_graph(&a.graphs[ref.inst]);
+                    c.inst = ref.inst;
+                }
+                if (a.inst_stats && (int)ref.inst != sinst) {
+                    if (sinst >= 0) flush_stats(c, &a.inst_stats[sinst]);
+                    sinst = ref.inst;
+                }
+                c.slot = ref.b1 > 0 ? (long long)ref.bound : (long long)hq;
+                // workspace: LDS with local matrices, LDS arrays with root
+                // matrices, slab with local matrices, or slab in root space
+                const bool fits = ref.t <= a.slab_tcap;
+                const bool local = ref.t <= (uint32_t)TL || (ref.t > wide && ref.t <= mid);
+                // (built per task: only one workspace descriptor is live)
+                if (ref.t <= (uint32_t)TL) {
+                    c.w = make_ws(lds, TL, true, (TL + 63) / 64);
+                    c.cap = TL;
+                } else if (ref.t <= wide) {
+                    c.w = make_ws_split(lds, wide,
+                                        a.lds_masks >= 0 ? (uint64_t*)(lds + a.lds_masks)
+                                                         : (uint64_t*)(slab + ws_bytes(a.slab_tcap, false, 0)),
+                                        a.slab_mwords);
+                    c.cap = wide;
+                } else if (ref.t <= mid) {
+                    c.w = make_ws(slab, mid, true, (mid + 63) / 64);
+                    if (a.lds_masks >= 0) {
+                        c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
+                        c.w.mk1 = c.w.mk0 + (mid + 63) / 64;
+                    }
+                    c.cap = mid;
+                } else {
+                    c.w = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
+                    if (a.lds_masks >= 0) {
+                        c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
+                        c.w.mk1 = c.w.mk0 + a.slab_mwords;
+                    }
+                    c.cap = a.slab_tcap;
+                }
+                if (!fits) {
+                    if (c.lane == 0) atomicOr(a.error, 2);
+                } else if (local) {
+                    process_task<true>(c, ref);
+                } else {
+                    process_task<false>(c, ref);
+                }
+                wsync();
+                if (a.timeline && c.lane == 0) {
+                    unsigned long long* r = a.timeline + 4 * hq;
+                    r[0] = (unsigned long long)tl_start;
+                    r[1] = (unsigned long long)wall_clock64();
+                    r[2] = ((unsigned long long)blockIdx.x << 32) | ref.t;
+                    r[3] = (unsigned long long)ref.depth | ((unsigned long long)ref.b0 << 16) |
+                           ((unsigned long long)ref.b1 << 40);
+                }
+                if (c.lane == 0)
+                    __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)-1ll, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                ph_mark(c, PH_CHILD);
             }
-            if (a.inst_stats && (int)ref.inst != sinst) {
-                if (sinst >= 0) flush_stats(c, &a.inst_stats[sinst]);
-                sinst = ref.inst;
-            }
-            c.slot = ref.b1 > 0 ? (long long)ref.bound : (long long)h;
-            // workspace: LDS with local matrices, LDS arrays with root
-            // matrices, slab with local matrices, or slab in root space
-            const bool fits = ref.t <= a.slab_tcap;
-            const bool local = ref.t <= (uint32_t)TL || (ref.t > wide && ref.t <= mid);
-            // (built per task: only one workspace descriptor is live)
-            if (ref.t <= (uint32_t)TL) {
-                c.w = make_ws(lds, TL, true, (TL + 63) / 64);
-                c.cap = TL;
-            } else if (ref.t <= wide) {
-                c.w = make_ws_split(lds, wide, (uint64_t*)(slab + ws_bytes(a.slab_tcap, false, 0)),
-                                    a.slab_mwords);
-                c.cap = wide;
-            } else if (ref.t <= mid) {
-                c.w = make_ws(slab, mid, true, (mid + 63) / 64);
-                c.cap = mid;
-            } else {
-                c.w = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
-                c.cap = a.slab_tcap;
-            }
-            if (!fits) {
-                if (c.lane == 0) atomicOr(a.error, 2);
-            } else if (local) {
-                process_task<true>(c, ref);
-            } else {
-                process_task<false>(c, ref);
-            }
-            wsync();
-            __threadfence();
-            if (c.lane == 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)-1ll);
-            ph_mark(c, PH_CHILD);
+            if (rd == 0) break;  // error 4: the next poll leaves
         }
     }
     ph_mark(c, PH_IDLE);

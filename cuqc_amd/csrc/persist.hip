@@ -100,6 +100,8 @@ int ensure(DBuf& b, size_t bytes) {
 
 // largest task that may use task-local matrices in the slab
 constexpr int MID_MAX = 4096;
+// largest pair of set masks kept in LDS (more would cost resident waves)
+constexpr size_t MASK_LDS_MAX = 4096;
 }  // namespace
 
 int enumerate_persistent(const Instance* const* insts, const qc_params* params, int count, EnumOut* outs,
@@ -163,15 +165,20 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     // wide tasks: per-vertex arrays in LDS up to this size (dynamic LDS)
     const int wide_req = p.lds_wide < 0 ? 0 : (p.lds_wide > 0 ? (int)p.lds_wide : 512);
     const uint32_t wide = wide_req > TL ? (uint32_t)std::min(std::min(wide_req, 1024), std::max(maxR, TL)) : 0u;
-    const size_t lds_b = std::max(ws_bytes(TL, true, (TL + 63) / 64), wide ? ws_bytes(wide, false, 0) : (size_t)0);
+    // the two set masks of tasks above the tile go to LDS after the wide
+    // arrays when they are small (rowpop reads the mask beside every row word)
+    const int mid_tile = std::min(p.mid_tile > 0 ? (int)p.mid_tile : 0, MID_MAX);
+    const uint32_t mid = (uint32_t)std::min(maxR, mid_tile);
+    const uint32_t midw = (mid + 63) / 64;
+    const size_t arr_b = wide ? ws_bytes(wide, false, 0) : (size_t)0;
+    const size_t mask_b = (size_t)2 * std::max<size_t>((size_t)maxrw, midw) * 8;
+    const bool mask_lds = mask_b <= MASK_LDS_MAX;
+    const size_t lds_b = std::max(ws_bytes(TL, true, (TL + 63) / 64), arr_b + (mask_lds ? mask_b : 0));
     QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds_b));
     // every wave must be resident: the queue hand-off waits on producers
     uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 16));
     if (p.workgroups > 0) grid = std::min<uint32_t>(grid, (uint32_t)p.workgroups);
 
-    const int mid_tile = std::min(p.mid_tile > 0 ? (int)p.mid_tile : 0, MID_MAX);
-    const uint32_t mid = (uint32_t)std::min(maxR, mid_tile);
-    const uint32_t midw = (mid + 63) / 64;
     const bool use_mid = (int)mid > TL;
     const size_t slab_b = std::max(ws_bytes_host((uint32_t)maxR, false, (uint32_t)maxrw),
                                    use_mid ? ws_bytes_host(mid, true, midw) : (size_t)0);
@@ -210,7 +217,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     }
 
     const bool trace = p.trace != 0;
-    DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_tbey, d_rfeat, d_cmw, d_cmn, d_phase;
+    DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_tbey, d_rfeat, d_cmw, d_cmn, d_phase, d_tl;
+    // diagnostics: QCMINE_TIMELINE=<file> writes one CSV line per task taken
+    const char* timeline_path = getenv("QCMINE_TIMELINE");
     if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
     if ((rc = ensure(ws.ctr, 10 * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
         (rc = ensure(ws.stats, sizeof(DevStats))) || (rc = ensure(ws.err, sizeof(int))) ||
@@ -280,11 +289,12 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.slab_mwords = (uint32_t)maxrw;
         A.mid_tcap = use_mid ? mid : 0;
         A.lds_wide = wide;
+        A.lds_masks = mask_lds ? (int32_t)arr_b : -1;
         A.stats = ws.stats.as<DevStats>();
         A.error = ws.err.as<int>();
         A.queue_mode = 1;
-        A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 16 : p.split_chunk);
-        A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : TL);
+        A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 32 : p.split_chunk);
+        A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : 512);
         A.q_refs = ws.q.as<TaskRef>();
         A.q_ready = ws.ready.as<unsigned int>();
         unsigned long long* ctr = ws.ctr.as<unsigned long long>();
@@ -302,6 +312,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.trace_parent = trace ? (long long*)d_tpar.p : nullptr;
         A.trace_beyond = trace ? d_tbey.as<unsigned int>() : nullptr;
         A.phase = phases ? d_phase.as<unsigned long long>() : nullptr;
+        if (timeline_path) {
+            if ((rc = d_tl.alloc(nb * q_cap * 32))) return rc;
+            QC_HIP(hipMemsetAsync(d_tl.p, 0, nb * q_cap * 32, s));
+            A.timeline = d_tl.as<unsigned long long>();
+        }
         QC_HIP(hipMemcpyAsync(ws.args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
         QC_HIP(hipStreamSynchronize(s));
 
@@ -408,6 +423,26 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
             return QC_EOVERFLOW;
         }
         break;
+    }
+    if (timeline_path) {
+        std::vector<unsigned long long> tl((size_t)nb * q_cap * 4);
+        QC_HIP(hipMemcpy(tl.data(), d_tl.p, tl.size() * 8, hipMemcpyDeviceToHost));
+        int khz = 0;
+        (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+        if (FILE* f = fopen(timeline_path, "w")) {
+            fprintf(f, "slot,start_us,end_us,wave,t,depth,b0,b1\n");
+            unsigned long long t0 = ~0ull;
+            for (size_t i = 0; i < tl.size() / 4; i++)
+                if (tl[4 * i + 1]) t0 = std::min(t0, tl[4 * i]);
+            const double us = khz > 0 ? 1000.0 / khz : 0.01;
+            for (size_t i = 0; i < tl.size() / 4; i++) {
+                const unsigned long long* r = &tl[4 * i];
+                if (!r[1]) continue;
+                fprintf(f, "%zu,%.3f,%.3f,%llu,%llu,%llu,%llu,%llu\n", i, (r[0] - t0) * us, (r[1] - t0) * us,
+                        r[2] >> 32, r[2] & 0xffffffffull, r[3] & 0xffffull, (r[3] >> 16) & 0xffffffull, r[3] >> 40);
+            }
+            fclose(f);
+        }
     }
     if (phases) {
         unsigned long long ph[PH_COUNT];

@@ -137,6 +137,8 @@ struct ExpandArgs {
                                      // this size use task-local matrices in the slab (0: none)
     uint32_t lds_wide;               // persistent mode: tasks above the LDS tile and up to
                                      // this size keep their per-vertex arrays in LDS (0: none)
+    int32_t lds_masks;               // persistent mode: byte offset in the dynamic LDS of the
+                                     // two set masks of tasks above the LDS tile (-1: slab)
 
     // persistent mode: n_buckets FIFO queues of task refs (bucket b owns
     // slots [b q_cap, (b+1) q_cap) of q_refs / q_ready and the counters
@@ -177,6 +179,9 @@ struct ExpandArgs {
     // diagnostics (NULL = off): shader cycles per phase summed over waves,
     // indexed by the PH_* constants
     unsigned long long* phase;
+    // diagnostics: per queue slot {start, end, wave << 32 | t, depth | b0 << 16 | b1 << 40}
+    // in wall-clock ticks, or NULL
+    unsigned long long* timeline;
 };
 
 enum PhaseId {

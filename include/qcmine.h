@@ -58,8 +58,8 @@ typedef struct qc_params {
     int32_t order;           /* qc_order */
     int32_t workgroups;      /* 0 = auto (persistent: CUs x occupancy) */
     int32_t lds_tile;        /* largest task (vertices) staged in LDS; 0 = auto */
-    int32_t split_chunk;     /* persistent mode: branches per work item when a task above the
-                                LDS tile is split across waves; 0 = auto, < 0 = never split */
+    int32_t split_chunk;     /* persistent mode: branches per work item when a task above
+                                split_min is split across waves; 0 = 32, < 0 = never split */
     int32_t max_depth;       /* search window: expand tasks of depth <= max_depth; <= 0: all */
     uint64_t task_capacity;  /* vertex slots per task pool; 0 = auto */
     uint64_t clique_capacity;/* raw-clique vertex slots; 0 = auto */
@@ -71,8 +71,8 @@ typedef struct qc_params {
                                 require identical counters from every replay; <= 1: once */
     int32_t trace;           /* persistent mode: record the cost-model features of every expanded
                                 task and its subtree size (qc_result_trace) */
-    int32_t split_min;       /* persistent mode: split only tasks above this many vertices; 0 = the
-                                LDS tile */
+    int32_t split_min;       /* persistent mode: split only tasks above this many vertices (and
+                                above the LDS tile); 0 = 512 */
     int32_t mid_tile;        /* persistent mode: tasks above the LDS tile and up to this size
                                 (<= 4096) build task-local bit matrices in HBM; <= 0 = none (read
                                 the root matrices) */

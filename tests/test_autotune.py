@@ -12,7 +12,7 @@ class _Res:
 
 def test_coordinate_descent_revisits_parameters(monkeypatch):
     # the best point needs split_chunk = -1 together with lds_wide = 0: a
-    # single sweep that fixes lds_wide first (under split_chunk 16) misses it
+    # single sweep that fixes lds_wide first (under the default chunk) misses it
     def fake_mine(g, ga, ms, sched, mode, window, **kw):
         t = 100.0
         if kw["split_chunk"] == -1:

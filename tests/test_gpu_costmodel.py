@@ -111,7 +111,7 @@ def _compare(res, ref):
 def test_cost_order_matches_oracle(cfg0_model, chunk):
     g, ref, model, _, _ = cfg0_model
     _say(f"[gpu] cost order config0 chunk={chunk}")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", order="cost", cost_model=model, split_chunk=chunk)
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", order="cost", cost_model=model, split_chunk=chunk, split_min=1)
     _compare(res, ref)
 
 
@@ -148,7 +148,7 @@ def test_cost_order_with_corrupted_model(cfg0_model, kind):
     else:
         bad.set_params(-p, nrm)
     _say(f"[gpu] cost order corrupted model {kind}")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", order="cost", cost_model=bad, split_chunk=4)
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", order="cost", cost_model=bad, split_chunk=4, split_min=1)
     _compare(res, ref)
 
 

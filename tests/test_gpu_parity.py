@@ -92,7 +92,7 @@ def test_config0_window(cfg0, window):
 def test_config0_persistent_split(cfg0, chunk):
     g, ref = cfg0
     _say(f"[gpu] config0 persistent split chunk={chunk}")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk)
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, split_min=1)
     _compare(res, ref, f"config0 split {chunk}")
 
 
@@ -102,7 +102,7 @@ def test_small_persistent_split(name, build, gamma, ms):
     _say(f"[gpu] small persistent split {name}")
     g = qc.Graph.from_csr(off, adj)
     ref = oracle.mine(off, adj, gamma, ms)
-    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, split_chunk=2)
+    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, split_chunk=2, split_min=1)
     _compare(res, ref, f"{name} split")
 
 
@@ -112,7 +112,7 @@ def test_config0_window_split(cfg0, window):
     off, adj = g.csr()
     _say(f"[gpu] config0 window split {window}")
     ref = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=window[0], root_end=window[1], max_depth=window[2])
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=4, window=window)
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=4, split_min=1, window=window)
     _compare(res, ref, f"config0 window {window} split")
 
 
@@ -124,7 +124,7 @@ def test_batched_small_instances():
     cases.append(("config0", *g0.csr(), g0.gamma, g0.min_size))
     _say(f"[gpu] batched {len(cases)} instances")
     graphs = [qc.Graph.from_csr(off, adj) for _, off, adj, _, _ in cases]
-    res = qc.mine_batch(graphs, [c[3] for c in cases], [c[4] for c in cases], lds_tile=64, split_chunk=4)
+    res = qc.mine_batch(graphs, [c[3] for c in cases], [c[4] for c in cases], lds_tile=64, split_chunk=4, split_min=1)
     for (name, off, adj, gamma, ms), r in zip(cases, res):
         _compare(r, oracle.mine(off, adj, gamma, ms), f"batched {name}")
 
@@ -154,7 +154,7 @@ def test_trace_matches_oracle(cfg0, chunk, window):
     ref = oracle.mine(off, adj, g.gamma, g.min_size, trace=True, root_begin=w[0], root_end=w[1], max_depth=w[2])
     sub = cm.subtree_sizes(ref["trace_parent"], ref["trace_beyond"])
     _say(f"[gpu] trace chunk={chunk} window={window}")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, trace=True,
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=chunk, split_min=1, trace=True,
                   window=window)
     assert len(res.trace_subtree) == ref["stats"]["tasks"]
     assert _trace_rows(res.trace_feats, res.trace_subtree) == _trace_rows(ref["trace_feats"], sub)
@@ -165,7 +165,7 @@ def test_capacity_growth_reruns(cfg0, monkeypatch):
     g, ref = cfg0
     monkeypatch.setenv("QCMINE_INITIAL_CAPACITY", "64")
     _say("[gpu] capacity growth from 64")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", split_chunk=4)
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", split_chunk=4, split_min=1)
     _compare(res, ref, "growth")
 
 
@@ -183,7 +183,7 @@ def test_config0_lds_wide(cfg0, lds_wide):
     """Tasks above the 64-vertex LDS tile with their arrays in LDS (or not)."""
     g, ref = cfg0
     _say(f"[gpu] config0 lds_wide={lds_wide}")
-    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, lds_wide=lds_wide, split_chunk=3)
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, lds_wide=lds_wide, split_chunk=3, split_min=1)
     _compare(res, ref, f"lds_wide {lds_wide}")
 
 
@@ -193,5 +193,5 @@ def test_small_lds_wide(name, build, gamma, ms):
     ref = oracle.mine(off, adj, gamma, ms)
     g = qc.Graph.from_csr(off, adj)
     _say(f"[gpu] small lds_wide {name}")
-    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, lds_wide=256, split_chunk=2)
+    res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, lds_wide=256, split_chunk=2, split_min=1)
     _compare(res, ref, name)
