@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE x1.0; profiles/hbm_traffic_coexpr8k_depth1.md).  Measured for the
 # schedule lds_tile 128 / split_chunk 32 / split_min 512 / lds_wide 0;
 # None where not measured.
-TRAFFIC: dict = {"coexpr8k": 69616632326}
+TRAFFIC: dict = {"coexpr8k": 38377515569}
 
 
 def _log(msg: str) -> None:
