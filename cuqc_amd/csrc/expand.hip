@@ -1508,26 +1508,17 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             w.cl2[nm] = w.pl2[vsrc];
         }
         int m = nm + 1, cc = nc - 1;
-        // pass 1: 2-hop survivors per 64-candidate chunk (masks in adjc)
-        uint64_t* smask = (uint64_t*)w.adjc;
-        int ns = 0;
-        for (int base = nm, k = 0; base < vsrc; base += WAVE, k++) {
-            const int q = base + c.lane;
-            const bool ok = q < vsrc && gbit(c.M2, c.W, vpp, PM(q));
-            const uint64_t bo = ballot(ok);
-            if (c.lane == 0) smask[k] = bo;
-            ns += __popcll(bo);
-        }
-        wsync();
-        // pass 2: place survivors (with indegrees in cind) and the others
+        // one pass over the candidates: the survivors of the 2-hop test go
+        // to [m, m + ns) in parent order with their degrees updated for v;
+        // the others (id and matrix index only) are staged in rem / rmv and
+        // appended after the survivors
         int nsd = 0, ndd = 0;
-        for (int base = nm, k = 0; base < vsrc; base += WAVE, k++) {
+        for (int base = nm; base < vsrc; base += WAVE) {
             const int q = base + c.lane;
             const bool in = q < vsrc;
-            const uint64_t bo = smask[k];
-            const bool ok = in && ((bo >> c.lane) & 1ull);
-            const uint64_t bx = ballot(in && !ok);
             const int pp = in ? PM(q) : 0;
+            const bool ok = in && gbit(c.M2, c.W, vpp, pp);
+            const uint64_t bo = ballot(ok), bx = ballot(in && !ok);
             int pexq = 0;
             if (in) {
                 // the removal of this branch (main.cu:1437-1450) for the
@@ -1548,13 +1539,20 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 w.cl2[d] = w.pl2[q];
                 w.cind[nsd + r] = cin_;
             } else if (in) {
-                const int d = m + ns + ndd + lanes_below(bx);
-                w.cid[d] = w.pid[q];
-                w.cpp[d] = pp;
-                w.clab[d] = -1;
+                const int d = ndd + lanes_below(bx);
+                w.rem[d] = w.pid[q];
+                w.rmv[d] = pp;
             }
             nsd += __popcll(bo);
             ndd += __popcll(bx);
+        }
+        const int ns = nsd;
+        wsync();
+        for (int k = c.lane; k < ndd; k += WAVE) {
+            const int d = m + ns + k;
+            w.cid[d] = w.rem[k];
+            w.cpp[d] = w.rmv[k];
+            w.clab[d] = -1;
         }
         int tc = m + ns + ndd;
         int nind = ns;
@@ -1702,8 +1700,10 @@ __global__ __launch_bounds__(64) void expand_hbm(const ExpandArgs* __restrict__ 
 // pending counter reaches zero; tasks of at most TL vertices run from LDS,
 // larger ones from the wave's HBM slab.
 // ---------------------------------------------------------------------------
+// (two resident waves per SIMD: the register allocation is held to 256)
 template <int TL>
-__global__ __launch_bounds__(64) void expand_persistent(const ExpandArgs* __restrict__ ap) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void expand_persistent(
+    const ExpandArgs* __restrict__ ap) {
     const ExpandArgs& a = *ap;
     // dynamic LDS: max(TL workspace with local matrices, arrays of wide tasks)
     extern __shared__ __attribute__((aligned(16))) char lds[];

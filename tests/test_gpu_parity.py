@@ -96,6 +96,19 @@ def test_config0_persistent_split(cfg0, chunk):
     _compare(res, ref, f"config0 split {chunk}")
 
 
+@pytest.mark.parametrize("order", ["fifo", "cost"])
+@pytest.mark.parametrize("workgroups", [8, 24])
+def test_config0_deep_queue_multi_claims(cfg0, order, workgroups):
+    """Few resident waves and one-branch items keep the queue many times
+    deeper than the grid, so waves claim 2 or 4 consecutive slots at once."""
+    g, ref = cfg0
+    _say(f"[gpu] config0 deep queue order={order} workgroups={workgroups}")
+    kw = dict(order="cost", cost_model=qc.CostModel.create(seed=3)) if order == "cost" else {}
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", lds_tile=64, split_chunk=1, split_min=1,
+                  workgroups=workgroups, **kw)
+    _compare(res, ref, f"config0 deep queue {order} {workgroups}")
+
+
 @pytest.mark.parametrize("name,build,gamma,ms", SMALL_CASES, ids=[c[0] for c in SMALL_CASES])
 def test_small_persistent_split(name, build, gamma, ms):
     off, adj = build()
