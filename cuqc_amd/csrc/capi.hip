@@ -5,7 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -13,6 +15,55 @@
 #include "qc_internal.h"
 
 namespace qc {
+
+// ---- device block cache (see DBuf) ----
+namespace {
+struct BlockCache {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void*>> free;
+};
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache;  // lives until the process ends
+    return *c;
+}
+}  // namespace
+
+void* pool_get(size_t cls, int dev, hipError_t* err) {
+    BlockCache& bc = block_cache();
+    {
+        std::lock_guard<std::mutex> lock(bc.mu);
+        auto it = bc.free.find({dev, cls});
+        if (it != bc.free.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            *err = hipSuccess;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {
+        // give the cached blocks of this device back and try once more
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lock(bc.mu);
+        for (auto& kv : bc.free)
+            if (kv.first.first == dev) {
+                for (void* q : kv.second) (void)hipFree(q);
+                kv.second.clear();
+            }
+        p = nullptr;
+        e = hipMalloc(&p, cls);
+    }
+    *err = e;
+    return e == hipSuccess ? p : nullptr;
+}
+
+void pool_put(void* p, size_t cls, int dev) {
+    BlockCache& bc = block_cache();
+    std::lock_guard<std::mutex> lock(bc.mu);
+    bc.free[{dev, cls}].push_back(p);
+}
+
 int load_serialized(const char* path, qc_graph* g);
 int load_adjlist(const char* path, qc_graph* g);
 int write_serialized(const qc_graph* g, const char* path);

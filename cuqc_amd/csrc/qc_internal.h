@@ -36,29 +36,59 @@ const char* last_error();
         }                                                                                      \
     } while (0)
 
+// Device blocks up to POOL_MAX bytes are recycled through a per-device
+// cache (power-of-two classes) instead of a hipMalloc / hipFree pair per
+// call; larger ones are allocated and freed directly.  A block goes back to
+// the cache only when its owner releases it, after the work using it has
+// been synchronized.
+constexpr size_t POOL_MAX = (size_t)64 << 20;
+inline size_t pool_class(size_t b) {
+    if (b > POOL_MAX) return 0;
+    size_t c = 256;
+    while (c < b) c <<= 1;
+    return c;
+}
+void* pool_get(size_t cls, int dev, hipError_t* err);
+void pool_put(void* p, size_t cls, int dev);
+
 // owning device buffer
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    size_t cls = 0;  // pool class, 0 = not pooled
+    int dev = 0;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (cls)
+                pool_put(p, cls, dev);
+            else
+                (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        cls = 0;
     }
     int alloc(size_t b) {
         release();
         if (b == 0) b = 16;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
+        (void)hipGetDevice(&dev);
+        const size_t c = pool_class(b);
+        hipError_t e = hipSuccess;
+        if (c)
+            p = pool_get(c, dev, &e);
+        else
+            e = hipMalloc(&p, b);
+        if (e != hipSuccess || !p) {
             p = nullptr;
             set_error("hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
             return QC_ENOMEM;
         }
         bytes = b;
+        cls = c;
         return QC_OK;
     }
     template <class T>

@@ -92,59 +92,103 @@ __global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off
 // The same rows and counts with the visited bitmap and both rows in LDS
 // (dynamic: ceil(n/64) + 2 rw words), for graphs whose bitmap fits.  Rows are
 // written whole, so A1 / A2 need no clearing.
+__device__ __forceinline__ uint64_t rdl64(uint64_t x, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <bool BYTES>
 __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                                     const int32_t* __restrict__ loc, const uint32_t* __restrict__ orig,
                                                     int32_t R, int32_t rw, int32_t n, uint64_t* A1, uint64_t* A2,
                                                     int32_t* n2out) {
+    // vis: {v} u N(v) u N(N(v)) over all n vertices, as a byte map (BYTES:
+    // plain stores of 1, no read-modify-write) or a bitmap (atomics without
+    // return); r1: root-space row of N(v).  |N2(v)| and the A2 row are read
+    // off the map afterwards.
     extern __shared__ unsigned long long sh[];
     const int lane = threadIdx.x & 63;
-    const int nw = (n + 63) / 64;
+    const int nw = BYTES ? (n + 7) / 8 : (n + 63) / 64;  // 64-bit words of the map
     unsigned long long* vis = sh;
+    unsigned char* vis8 = (unsigned char*)sh;
     unsigned long long* r1 = sh + nw;
-    unsigned long long* r2 = r1 + rw;
+    auto mark = [&](uint32_t x) {
+        if (BYTES)
+            vis8[x] = 1;
+        else
+            atomicOr(&vis[x >> 6], 1ull << (x & 63));
+    };
+    auto seen = [&](uint32_t x) -> bool { return BYTES ? vis8[x] != 0 : ((vis[x >> 6] >> (x & 63)) & 1ull); };
+    constexpr int G = 8;  // neighbours u whose lists are walked together
     for (int a = blockIdx.x; a < R; a += gridDim.x) {
         const uint32_t v = orig[a];
         for (int w = lane; w < nw; w += 64) vis[w] = 0;
-        for (int w = lane; w < rw; w += 64) {
-            r1[w] = 0;
-            r2[w] = 0;
-        }
+        for (int w = lane; w < rw; w += 64) r1[w] = 0;
         __syncthreads();
-        if (lane == 0) atomicOr(&vis[v >> 6], 1ull << (v & 63));
-        __syncthreads();
+        if (lane == 0) mark(v);
         const uint64_t b0 = off[v], b1 = off[v + 1];
-        int cnt = 0;
         for (uint64_t j = b0 + lane; j < b1; j += 64) {
             const uint32_t u = adj[j];
-            const unsigned long long bit = 1ull << (u & 63);
-            const unsigned long long old = atomicOr(&vis[u >> 6], bit);
+            mark(u);
             const int la = loc[u];
             if (la >= 0) atomicOr(&r1[la >> 6], 1ull << (la & 63));
-            if (!(old & bit)) {
-                cnt++;
-                if (la >= 0) atomicOr(&r2[la >> 6], 1ull << (la & 63));
+        }
+        // N(u) for G neighbours u at a time as one flat range of entries:
+        // lane i < G holds u_i's list start and the prefix of list lengths
+        for (uint64_t j0 = b0; j0 < b1; j0 += G) {
+            uint64_t st = 0, len = 0;
+            if (lane < G && j0 + lane < b1) {
+                const uint32_t u = adj[j0 + lane];
+                st = off[u];
+                len = off[u + 1] - st;
+            }
+            uint64_t pre = len;  // inclusive prefix over lanes 0..G-1
+            for (int o = 1; o < G; o <<= 1) {
+                const uint64_t y = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += y;
+            }
+            // segment ends and starts as wave-uniform values
+            uint64_t e[G], sv[G];
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                e[q] = rdl64(pre, q);
+                sv[q] = rdl64(st, q);
+            }
+            const uint64_t total = e[G - 1];
+#pragma unroll 4
+            for (uint64_t p = lane; p < total; p += 64) {
+                uint64_t s0 = sv[0], base = 0;
+#pragma unroll
+                for (int q = 0; q < G - 1; q++)
+                    if (p >= e[q]) {
+                        s0 = sv[q + 1];
+                        base = e[q];
+                    }
+                mark(adj[s0 + (p - base)]);
             }
         }
         __syncthreads();
-        for (uint64_t j = b0; j < b1; j++) {
-            const uint32_t u = adj[j];
-            for (uint64_t k = off[u] + lane; k < off[u + 1]; k += 64) {
-                const uint32_t x = adj[k];
-                const unsigned long long bit = 1ull << (x & 63);
-                const unsigned long long old = atomicOr(&vis[x >> 6], bit);
-                if (!(old & bit)) {
-                    cnt++;
-                    const int la = loc[x];
-                    if (la >= 0) atomicOr(&r2[la >> 6], 1ull << (la & 63));
-                }
-            }
-        }
+        int cnt = 0;
+        for (int w = lane; w < nw; w += 64) cnt += __popcll(vis[w]);  // bytes are 0 or 1
         for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-        __syncthreads();
-        if (lane == 0) n2out[a] = cnt;
-        for (int w = lane; w < rw; w += 64) {
-            A1[(size_t)a * rw + w] = r1[w];
-            A2[(size_t)a * rw + w] = r2[w];
+        if (lane == 0) n2out[a] = cnt - 1;  // v itself is in the bitmap
+        for (int w = lane; w < rw; w += 64) A1[(size_t)a * rw + w] = r1[w];
+        // A2 row: bit b iff orig[b] was reached (b != a), one ballot per
+        // word; eight words per step keep eight orig loads in flight
+        for (int w0 = 0; w0 < rw; w0 += 8) {
+            uint32_t xs[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int b = (w0 + k) * 64 + lane;
+                xs[k] = (w0 + k < rw && b < R) ? orig[b] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int b = (w0 + k) * 64 + lane;
+                const unsigned long long word = __ballot(xs[k] != 0xFFFFFFFFu && b != a && seen(xs[k]));
+                if (lane == 0 && w0 + k < rw) A2[(size_t)a * rw + w0 + k] = word;
+            }
         }
         __syncthreads();
     }
@@ -330,16 +374,20 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     const size_t mat_bytes = (size_t)R * I.rw * 8;
     if ((rc = I.A1.alloc(mat_bytes))) return rc;
     if ((rc = I.A2.alloc(mat_bytes))) return rc;
-    // LDS form: visited bitmap + both rows of one root vertex per wave
-    const size_t lds_bytes = (((size_t)n + 63) / 64 + 2 * (size_t)I.rw) * 8;
+    // LDS form: visited map + the 1-hop row of one root vertex per wave; a
+    // byte map (plain stores) up to 32K vertices, a bitmap (atomics) above
+    const bool byte_map = n <= 32768;
+    const size_t map_words = byte_map ? ((size_t)n + 7) / 8 : ((size_t)n + 63) / 64;
+    const size_t lds_bytes = (map_words + (size_t)I.rw) * 8;
+    const void* rows_kernel =
+        byte_map ? (const void*)qcdev::root_rows_lds<true> : (const void*)qcdev::root_rows_lds<false>;
     int lds_max = 0, cur_dev = 0;
     QC_HIP(hipGetDevice(&cur_dev));
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, cur_dev) != hipSuccess)
         lds_max = 64 * 1024;
     bool lds_rows = !g->has_n2 && lds_bytes <= (size_t)lds_max;
     if (lds_rows && lds_bytes > 48 * 1024 &&
-        hipFuncSetAttribute((const void*)qcdev::root_rows_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes) != hipSuccess)
+        hipFuncSetAttribute(rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
         lds_rows = false;
     if (!lds_rows) {
         QC_HIP(hipMemsetAsync(I.A1.p, 0, mat_bytes, s));
@@ -355,14 +403,28 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     QC_HIP(hipMemcpyAsync(I.d_orig.p, I.orig.data(), (size_t)R * 4, hipMemcpyHostToDevice, s));
     int blocks = std::min(R, 2048);
     uint64_t vis_words = ((uint64_t)n + 63) / 64;
+    const bool phases = getenv("QCMINE_PHASES") != nullptr;
+    auto stamp = [&]() {
+        if (phases) (void)hipStreamSynchronize(s);
+        return std::chrono::steady_clock::now();
+    };
+    const auto t_up = stamp();
     if (lds_rows) {
         int dev = 0, n_cu = 0;
         QC_HIP(hipGetDevice(&dev));
         QC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
         const int lblocks = std::min(R, std::max(1, n_cu) * 16);
-        hipLaunchKernelGGL(qcdev::root_rows_lds, dim3(lblocks), dim3(64), lds_bytes, s, G->off.as<uint64_t>(),
-                           G->adj.as<uint32_t>(), d_loc.as<int32_t>(), I.d_orig.as<uint32_t>(), R, I.rw, n,
-                           I.A1.as<uint64_t>(), I.A2.as<uint64_t>(), d_n2.as<int32_t>());
+        const uint64_t* offp = G->off.as<uint64_t>();
+        const uint32_t* adjp = G->adj.as<uint32_t>();
+        const int32_t* locp = d_loc.as<int32_t>();
+        const uint32_t* origp = I.d_orig.as<uint32_t>();
+        int32_t Rv = R, rwv = I.rw, nv = n;
+        uint64_t* A1p = I.A1.as<uint64_t>();
+        uint64_t* A2p = I.A2.as<uint64_t>();
+        int32_t* n2p = d_n2.as<int32_t>();
+        void* rargs[] = {(void*)&offp, (void*)&adjp, (void*)&locp, (void*)&origp, (void*)&Rv,
+                         (void*)&rwv,  (void*)&nv,   (void*)&A1p,  (void*)&A2p,   (void*)&n2p};
+        QC_HIP(hipLaunchKernel(rows_kernel, dim3(lblocks), dim3(64), rargs, lds_bytes, s));
     } else {
         if (!g->has_n2) {
             if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;
@@ -374,6 +436,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
                            d_n2.as<int32_t>(), d_vis.as<unsigned long long>(), vis_words, g->has_n2 ? 1 : 0);
     }
     QC_HIP(hipGetLastError());
+    const auto t_rows = stamp();
     if ((rc = I.d_d2r.alloc((size_t)R * 4))) return rc;
     if ((rc = I.d_d1r.alloc((size_t)R * 4))) return rc;
     hipLaunchKernelGGL(qcdev::root_d2, dim3(blocks), dim3(64), 0, s, I.A2.as<uint64_t>(), I.A1.as<uint64_t>(), R,
@@ -388,11 +451,14 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     QC_HIP(hipMemcpyAsync(I.d_mindeg.p, I.mindeg.data(), I.mindeg.size() * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(I.d_ubq.p, I.ubq.data(), I.ubq.size() * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipStreamSynchronize(s));
-    if (getenv("QCMINE_PHASES")) {
+    if (phases) {
         const auto t_dev = std::chrono::steady_clock::now();
-        fprintf(stderr, "[qcmine] root: R=%d host filter/peel %.2f ms (from call), device rows %.2f ms (%s)\n", R,
-                std::chrono::duration<double, std::milli>(t_host - t_start).count(),
-                std::chrono::duration<double, std::milli>(t_dev - t_host).count(), lds_rows ? "lds" : "global");
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        fprintf(stderr,
+                "[qcmine] root: R=%d host filter/peel %.2f ms (from call), device rows %.2f ms (%s; uploads %.2f, "
+                "rows %.2f, degrees and tables %.2f)\n",
+                R, ms(t_start, t_host), ms(t_host, t_dev), lds_rows ? "lds" : "global", ms(t_host, t_up),
+                ms(t_up, t_rows), ms(t_rows, t_dev));
     }
 
     // ---- cover vertex and covered set (main.cu:953-975) ----
