@@ -1832,6 +1832,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                     if (a.lds_masks >= 0) {
                         c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
                         c.w.mk1 = c.w.mk0 + a.slab_mwords;
+                        // the LDS below the masks (the wide tier's arrays) is
+                        // free during a slab task: the child's indegree,
+                        // exdegree and label arrays, read by every pruning
+                        // pass, go there when they fit
+                        const uint32_t T2t = pow2_at_least(ref.t < 64 ? 64 : ref.t);
+                        if ((size_t)3 * T2t * 4 <= (size_t)a.lds_masks) {
+                            int* lp = (int*)lds;
+                            c.w.cin = lp;
+                            c.w.cex = lp + T2t;
+                            c.w.clab = lp + 2 * T2t;
+                        }
                     }
                     c.cap = a.slab_tcap;
                 }
