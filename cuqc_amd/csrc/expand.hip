@@ -1833,16 +1833,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                         c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
                         c.w.mk1 = c.w.mk0 + a.slab_mwords;
                         // the LDS below the masks (the wide tier's arrays) is
-                        // free during a slab task: the child's indegree,
-                        // exdegree and label arrays, read by every pruning
-                        // pass, go there when they fit
+                        // free during a slab task: as many of the child's
+                        // arrays as fit go there, the ones every pruning pass
+                        // reads first (indegree, exdegree, label, then id,
+                        // matrix index, lvl2adj)
                         const uint32_t T2t = pow2_at_least(ref.t < 64 ? 64 : ref.t);
-                        if ((size_t)3 * T2t * 4 <= (size_t)a.lds_masks) {
-                            int* lp = (int*)lds;
-                            c.w.cin = lp;
-                            c.w.cex = lp + T2t;
-                            c.w.clab = lp + 2 * T2t;
-                        }
+                        const uint32_t k = (uint32_t)a.lds_masks / (T2t * 4);
+                        int* lp = (int*)lds;
+                        if (k >= 1) c.w.cin = lp;
+                        if (k >= 2) c.w.cex = lp + T2t;
+                        if (k >= 3) c.w.clab = lp + 2 * T2t;
+                        if (k >= 4) c.w.cid = lp + 3 * T2t;
+                        if (k >= 5) c.w.cpp = lp + 4 * T2t;
+                        if (k >= 6) c.w.cl2 = lp + 5 * T2t;
                     }
                     c.cap = a.slab_tcap;
                 }
