@@ -1837,15 +1837,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                         // arrays as fit go there, the ones every pruning pass
                         // reads first (indegree, exdegree, label, then id,
                         // matrix index, lvl2adj)
-                        const uint32_t T2t = pow2_at_least(ref.t < 64 ? 64 : ref.t);
-                        const uint32_t k = (uint32_t)a.lds_masks / (T2t * 4);
+                        // (child positions stay below t: a stride of t rounded
+                        // up to 64 entries suffices)
+                        const uint32_t Tt = (ref.t + 63u) & ~63u;
+                        const uint32_t k = (uint32_t)a.lds_masks / (Tt * 4);
                         int* lp = (int*)lds;
                         if (k >= 1) c.w.cin = lp;
-                        if (k >= 2) c.w.cex = lp + T2t;
-                        if (k >= 3) c.w.clab = lp + 2 * T2t;
-                        if (k >= 4) c.w.cid = lp + 3 * T2t;
-                        if (k >= 5) c.w.cpp = lp + 4 * T2t;
-                        if (k >= 6) c.w.cl2 = lp + 5 * T2t;
+                        if (k >= 2) c.w.cex = lp + Tt;
+                        if (k >= 3) c.w.clab = lp + 2 * Tt;
+                        if (k >= 4) c.w.cid = lp + 3 * Tt;
+                        if (k >= 5) c.w.cpp = lp + 4 * Tt;
+                        if (k >= 6) c.w.cl2 = lp + 5 * Tt;
                     }
                     c.cap = a.slab_tcap;
                 }
