@@ -148,17 +148,20 @@ __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__
                 const uint64_t y = __shfl_up(pre, o, 64);
                 if (lane >= o) pre += y;
             }
-            // segment ends and starts as wave-uniform values
-            uint64_t e[G], sv[G];
+            // segment ends and starts as wave-uniform values (the G lists
+            // of one group hold far fewer than 2^32 entries)
+            uint32_t e[G];
+            uint64_t sv[G];
 #pragma unroll
             for (int q = 0; q < G; q++) {
-                e[q] = rdl64(pre, q);
+                e[q] = (uint32_t)rdl64(pre, q);
                 sv[q] = rdl64(st, q);
             }
-            const uint64_t total = e[G - 1];
-#pragma unroll 4
-            for (uint64_t p = lane; p < total; p += 64) {
-                uint64_t s0 = sv[0], base = 0;
+            const uint32_t total = e[G - 1];
+#pragma unroll 8
+            for (uint32_t p = lane; p < total; p += 64) {
+                uint64_t s0 = sv[0];
+                uint32_t base = 0;
 #pragma unroll
                 for (int q = 0; q < G - 1; q++)
                     if (p >= e[q]) {
@@ -410,10 +413,9 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     };
     const auto t_up = stamp();
     if (lds_rows) {
-        int dev = 0, n_cu = 0;
-        QC_HIP(hipGetDevice(&dev));
-        QC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-        const int lblocks = std::min(R, std::max(1, n_cu) * 16);
+        // one block per root vertex: the dispatcher hands the next vertex to
+        // whichever slot frees first (their costs differ by orders of magnitude)
+        const int lblocks = std::min(R, 1 << 20);
         const uint64_t* offp = G->off.as<uint64_t>();
         const uint32_t* adjp = G->adj.as<uint32_t>();
         const int32_t* locp = d_loc.as<int32_t>();
