@@ -402,19 +402,26 @@ __device__ void sort_keys(Ctx& c, uint64_t* hi, uint64_t* lo, int n) {
 // Returns V = the largest value, or -1 when the tables do not fit (the
 // caller sorts).
 // ---------------------------------------------------------------------------
+// One chunk of candidate indegrees (x < 0: no value in this lane) into the
+// register table of the m < 64 case: lane v counts the values >= v, V is the
+// largest value.  Loops that produce the indegrees call it on the values in
+// their registers, so the table costs no pass of its own.
+__device__ __forceinline__ void hist_chunk(const Ctx& c, int x, int& creg, int& V) {
+    const int top = uni(wave_max(x));
+    V = max(V, top);
+    for (int v = 1; v <= top; v++) {
+        const int k = __popcll(ballot(x >= v));
+        if (c.lane == v) creg += k;
+    }
+}
+
 __device__ int indeg_dist(Ctx& c, int n, int m, int& creg, bool& reg) {
     WS& w = c.w;
     if (m < WAVE) {
         int cnt = 0, V = 0;
         for (int base = 0; base < n; base += WAVE) {
             const int i = base + c.lane;
-            const int x = i < n ? w.cind[i] : -1;
-            const int top = uni(wave_max(x));
-            V = max(V, top);
-            for (int v = 1; v <= top; v++) {
-                const int k = __popcll(ballot(x >= v));
-                if (c.lane == v) cnt += k;
-            }
+            hist_chunk(c, i < n ? w.cind[i] : -1, cnt, V);
         }
         creg = cnt;
         reg = true;
@@ -587,15 +594,21 @@ __device__ inline bool members_fail(Ctx& c, int m, int ub, int lb, int med) {
 // degree pruning, main.cu:1735-1855.  cind[0..nind) = candidate indegrees.
 // ---------------------------------------------------------------------------
 // map_empty: the reference's id->position map holds no entry during this call
-// (critical-vertex pruning without critical-adjacent candidates).
+// (critical-vertex pruning without critical-adjacent candidates).  pre_V >= 0:
+// the caller built the register table of cind[0..nind) (pre_V, pre_creg).
 __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, int& nc, int m, int& ub, int& lb,
-                                                              int& med, int nind, bool map_empty) {
+                                                              int& med, int nind, bool map_empty, int pre_V,
+                                                              int pre_creg) {
     WS& w = c.w;
     ph_mark(c, PH_PRUNE);
-    int creg = 0;
-    bool reg = false;
-    int V = indeg_dist(c, nind, m, creg, reg);
-    if (V < 0) sort_desc_int(c, w.cind, nind);
+    int creg = pre_creg;
+    bool reg = pre_V >= 0;
+    int V = pre_V;
+    if (!reg) {
+        V = indeg_dist(c, nind, m, creg, reg);
+        if (V < 0) sort_desc_int(c, w.cind, nind);
+    }
+    ph_mark(c, PH_HIST);
     if (lu_bounds(c, m, nind, ub, lb, med, V, creg, reg) || members_fail(c, m, ub, lb, med)) {
         ph_mark(c, PH_LU);
         return true;
@@ -647,7 +660,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             for (int k = c.lane; k < nl; k += WAVE) set_mask_bit(w.mk0, w.cpp[L[k]]);
             wsync();
         }
-        int nv = 0;
+        int nv = 0, hV = 0, hc = 0;
         for (int base = 0; base < nupd; base += WAVE) {
             const int j = base + c.lane;
             bool ok = false;
@@ -674,14 +687,23 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
                 }
             }
             const uint64_t bo = ballot(ok);
-            if (ok) w.cind[nv + lanes_below(bo)] = in;
+            // (with m < 64 the register table replaces the list)
+            if (ok && m >= WAVE) w.cind[nv + lanes_below(bo)] = in;
             nv += __popcll(bo);
+            if (m < WAVE) hist_chunk(c, ok ? in : -1, hc, hV);
         }
         wsync();
         ph_mark(c, PH_DEGUPD);
         ph_mark(c, PH_COMPACT);
-        V = indeg_dist(c, nv, m, creg, reg);
-        if (V < 0) sort_desc_int(c, w.cind, nv);
+        if (m < WAVE) {
+            V = hV;
+            creg = hc;
+            reg = true;
+        } else {
+            V = indeg_dist(c, nv, m, creg, reg);
+            if (V < 0) sort_desc_int(c, w.cind, nv);
+        }
+        ph_mark(c, PH_HIST);
         if (lu_bounds(c, m, nv, ub, lb, med, V, creg, reg) || members_fail(c, m, ub, lb, med)) {
             ph_mark(c, PH_LU);
             return true;
@@ -1512,7 +1534,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // to [m, m + ns) in parent order with their degrees updated for v;
         // the others (id and matrix index only) are staged in rem / rmv and
         // appended after the survivors
-        int nsd = 0, ndd = 0;
+        int nsd = 0, ndd = 0, hV = 0, hc = 0;
         for (int base = nm; base < vsrc; base += WAVE) {
             const int q = base + c.lane;
             const bool in = q < vsrc;
@@ -1526,18 +1548,20 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 pexq = w.pex[q];
                 if (pend >= 0 && gbit(c.M1, c.W, pend, pp)) w.pex[q] = --pexq;
             }
+            int xin = -1;
             if (ok) {
                 const int r = lanes_below(bo);
                 const int d = m + nsd + r;
                 const int a1 = (int)gbit(c.M1, c.W, vpp, pp);
                 const int cin_ = w.pin[q] + a1;
+                xin = cin_;
                 w.cid[d] = w.pid[q];
                 w.cpp[d] = pp;
                 w.clab[d] = 0;
                 w.cin[d] = cin_;
                 w.cex[d] = pexq - a1;
                 w.cl2[d] = w.pl2[q];
-                w.cind[nsd + r] = cin_;
+                if (m >= WAVE) w.cind[nsd + r] = cin_;  // else the register table
             } else if (in) {
                 const int d = ndd + lanes_below(bx);
                 w.rem[d] = w.pid[q];
@@ -1545,6 +1569,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             }
             nsd += __popcll(bo);
             ndd += __popcll(bx);
+            if (m < WAVE) hist_chunk(c, xin, hc, hV);
         }
         const int ns = nsd;
         wsync();
@@ -1563,8 +1588,10 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // second degree pruning (one call site: the call is inlined)
         bool first = true, map_empty = false, skip = false;
         int r = 0;
+        int preV = m < WAVE ? hV : -1;  // the build's table of the survivors' indegrees
         for (;;) {
-            const bool fail = degree_pruning(c, tc, cc, m, ub, lb, med, nind, map_empty);
+            const bool fail = degree_pruning(c, tc, cc, m, ub, lb, med, nind, map_empty, preV, hc);
+            preV = -1;
             if (!first) {
                 r = fail ? 1 : 0;
                 break;

@@ -448,7 +448,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         unsigned long long ph[PH_COUNT];
         QC_HIP(hipMemcpy(ph, d_phase.p, sizeof(ph), hipMemcpyDeviceToHost));
         static const char* names[] = {"load", "lookahead", "remove", "add", "prune", "child", "prepare", "idle",
-                                      "lu", "degupd", "classify", "critical", "compact"};
+                                      "lu", "degupd", "classify", "critical", "compact", "hist"};
         double tot = 0;
         for (int k = 0; k < PH_COUNT; k++) tot += (double)ph[k];
         fprintf(stderr, "[qcmine] phases (grid %u waves, %.3g wave-cycles):", grid, tot);

@@ -193,10 +193,11 @@ enum PhaseId {
     PH_CHILD,         // child sort and write
     PH_PREPARE,       // split preparation
     PH_IDLE,          // persistent pop loop: waiting for work
-    PH_LU,            // candidate indegree tables, LU bounds, member checks
+    PH_LU,            // LU bounds and member checks
     PH_DEGUPD,        // degree-pruning exdeg updates
     PH_COMPACT,       // degree-pruning classification
     PH_CRIT,          // critical-vertex step
     PH_MOVE,          // degree-pruning compaction of the kept candidates
+    PH_HIST,          // candidate indegree tables for the LU bounds
     PH_COUNT
 };
