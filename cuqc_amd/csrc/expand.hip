@@ -1341,6 +1341,14 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
 
     int b_lo = 0, b_hi = nb;
     bool count_removes = true;
+    // a root's branches are split finer, to about one item per resident wave
+    // over all roots: until the roots' items produce children nothing else
+    // can run
+    int ch = a.split_chunk;
+    if (ref.depth == 0 && ch > 0 && a.n_roots > 0) {
+        const long long want = ((long long)(bend - bbeg) * a.n_roots + gridDim.x - 1) / gridDim.x;
+        ch = (int)max(1ll, min((long long)ch, want));
+    }
     if (item) {
         // split item: apply removals 1..b0 at once (the prepare step found no
         // failure before its range), then run branches [b0, b1); b0 and b1
@@ -1355,7 +1363,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         b_lo = b0;
         b_hi = (int)ref.b1;
         count_removes = false;
-    } else if (a.queue_mode && a.split_chunk > 0 && (uint32_t)t0 > a.split_min_t && bend - bbeg > a.split_chunk) {
+    } else if (a.queue_mode && ch > 0 && (uint32_t)t0 > a.split_min_t && bend - bbeg > ch) {
         // prepare step: find the first failing removal, count the
         // removals' scans, store the record as it stands before the first
         // branch of the window (with lvl2adj) and push the items
@@ -1407,7 +1415,6 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             wadj_lanes(c, rank + (c.lane == 0));
         }
         const int hi = min(kb, bend);
-        const int ch = a.split_chunk;
         if (hi <= bbeg) {
             ph_mark(c, PH_PREPARE);
             return;
@@ -1770,12 +1777,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                     unsigned long long tb =
                         __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (hb < tb && hb < a.q_cap) {
-                        // a deep queue hands out several consecutive slots
-                        // per claim: one head update per claim limits how
-                        // fast tiny tasks can be taken
+                        // a very deep queue hands out several consecutive
+                        // slots per claim: one head update per claim limits
+                        // how fast tiny tasks can be taken.  Only very deep:
+                        // the slots of one claim run one after another, which
+                        // lengthens the critical path when they are heavy
                         const unsigned long long avail = min(tb, a.q_cap) - hb;
                         const unsigned long long grid = gridDim.x;
-                        const int k = avail >= 8 * grid ? 4 : avail >= 4 * grid ? 2 : 1;
+                        const int k = avail >= 32 * grid ? 4 : avail >= 16 * grid ? 2 : 1;
                         if (atomicCAS(hp, hb, hb + k) == hb) {
                             state = 1;
                             h = hb + (unsigned long long)b * a.q_cap;

@@ -303,6 +303,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.pool_top = ctr + 2 * nb * QSTRIDE;
         A.pending = (long long*)(ctr + (2 * nb + 1) * QSTRIDE);
         A.n_buckets = nb;
+        A.n_roots = (int32_t)roots.size();
         A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
         A.cm_norm = cost ? d_cmn.as<float>() : nullptr;
         A.q_cap = q_cap;

@@ -169,6 +169,7 @@ struct ExpandArgs {
     unsigned int* trace_beyond;  // per slot: children past the depth window
 
     int32_t n_buckets;               // 1 = FIFO
+    int32_t n_roots;                 // root tasks of the launch (root split width)
     int32_t pad2;
     const uint16_t* cm_w;            // cost model: bf16 weights (5057)
     const float* cm_norm;            // mu[12] sd[12] ymu ysd thresholds[3]
