@@ -239,6 +239,8 @@ struct Ctx {
     long long ph_t;                   // start of the running phase
     long long wadj_l;                 // this lane's share of W_adj (summed at flush)
     bool phase_on;                    // phase counters requested
+    bool keys;                        // the task's children are written (inside the depth
+                                      // window): only then is the child's lvl2adj kept
 };
 
 // phase accounting: charge the cycles since the last mark to phase k
@@ -741,7 +743,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             lab = w.clab[i];
             in = w.cin[i];
             ex = w.cex[i];
-            l2 = w.cl2[i];
+            if (c.keys) l2 = w.cl2[i];
         }
         wsync();
         if (k < nrem) {
@@ -751,7 +753,7 @@ __device__ __attribute__((always_inline)) bool degree_pruning(Ctx& c, int& tc, i
             w.clab[d] = lab;
             w.cin[d] = in;
             w.cex[d] = ex;
-            w.cl2[d] = l2;
+            if (c.keys) w.cl2[d] = l2;
         }
         wsync();
     }
@@ -854,7 +856,7 @@ __device__ __attribute__((always_inline)) int critical_prep(Ctx& c, int tc, int&
         permute(w.clab);
         permute(w.cin);
         permute(w.cex);
-        permute(w.cl2);
+        if (c.keys) permute(w.cl2);
     }
     // add the critical-adjacent vertices: degree updates and 2-hop counts
     const bool sparse = ncrit < c.W;
@@ -1145,6 +1147,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
     WS& w = c.w;
     const int t0 = (int)ref.t;
     c.depth = ref.depth;
+    c.keys = !(c.g.max_depth >= 0 && (int)c.depth + 1 > c.g.max_depth);
     const bool item = ref.b1 > 0;
     if (!item) {  // a split item's task is counted by its prepare step
         if (c.lane == 0) {
@@ -1524,7 +1527,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             w.clab[i] = w.plab[i];
             w.cin[i] = w.pin[i] + a1;
             w.cex[i] = w.pex[i] - a1;
-            w.cl2[i] = w.pl2[i];
+            if (c.keys) w.cl2[i] = w.pl2[i];
         }
         if (c.lane == 0) {
             int pv = w.pex[vsrc];
@@ -1534,7 +1537,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             w.clab[nm] = 1;
             w.cin[nm] = w.pin[vsrc];
             w.cex[nm] = pv;
-            w.cl2[nm] = w.pl2[vsrc];
+            if (c.keys) w.cl2[nm] = w.pl2[vsrc];
         }
         int m = nm + 1, cc = nc - 1;
         // one pass over the candidates: the survivors of the 2-hop test go
@@ -1567,7 +1570,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 w.clab[d] = 0;
                 w.cin[d] = cin_;
                 w.cex[d] = pexq - a1;
-                w.cl2[d] = w.pl2[q];
+                if (c.keys) w.cl2[d] = w.pl2[q];
                 if (m >= WAVE) w.cind[nsd + r] = cin_;  // else the register table
             } else if (in) {
                 const int d = ndd + lanes_below(bx);
