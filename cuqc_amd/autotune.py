@@ -20,7 +20,9 @@ it):
 The search is coordinate descent from the library defaults, one parameter at
 a time, repeated while a full sweep improves the time by more than
 ``min_gain`` (at most ``max_sweeps`` sweeps), ``reps`` timed runs per
-candidate (median kernel time).  Every trial
+candidate (median kernel time).  A winner other than the defaults is timed
+again against them (three runs each) and kept only if it still wins by
+``min_gain``.  Every trial
 must reproduce the reference schedule's counters (tasks, branches, raw
 cliques, adjacency scans) and raw-set digest: a schedule changes the order of
 work, never the result, and a trial that disagrees raises.
@@ -142,6 +144,21 @@ def tune(graphs, gammas, min_sizes, windows=None, *, reps: int = 2, order: str =
                     improved = True
         if not improved:
             break
+    # the search times each point once or twice; before leaving the library
+    # defaults, time the winner and the defaults again, three runs each, and
+    # keep the winner only if it still beats them by min_gain
+    if best != DEFAULT:
+
+        def again(cfg: dict) -> float:
+            return float(np.median([max(r.stats["t_kernel_ms"] for r in run(cfg)) for _ in range(3)]))
+
+        t_best, t_def = again(best), again(DEFAULT)
+        if log:
+            log(f"[autotune] confirm {best} -> {t_best:.2f} ms against defaults {t_def:.2f} ms")
+        if t_best < t_def * (1.0 - min_gain):
+            best_t = t_best
+        else:
+            best, best_t = dict(DEFAULT), t_def
     sched = Schedule(kernel_ms=round(best_t, 3), trials=trials, **best)
     if cache is not None:
         known = {}

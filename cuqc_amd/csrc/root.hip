@@ -92,6 +92,8 @@ __global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off
 // The same rows and counts with the visited bitmap and both rows in LDS
 // (dynamic: ceil(n/64) + 2 rw words), for graphs whose bitmap fits.  Rows are
 // written whole, so A1 / A2 need no clearing.
+constexpr int ROWS_THREADS = 256;
+
 __device__ __forceinline__ uint64_t rdl64(uint64_t x, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
@@ -99,7 +101,7 @@ __device__ __forceinline__ uint64_t rdl64(uint64_t x, int l) {
 }
 
 template <bool BYTES>
-__global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+__global__ __launch_bounds__(ROWS_THREADS) void root_rows_lds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                                     const int32_t* __restrict__ loc, const uint32_t* __restrict__ orig,
                                                     int32_t R, int32_t rw, int32_t n, uint64_t* A1, uint64_t* A2,
                                                     int32_t* n2out) {
@@ -107,8 +109,13 @@ __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__
     // plain stores of 1, no read-modify-write) or a bitmap (atomics without
     // return); r1: root-space row of N(v).  |N2(v)| and the A2 row are read
     // off the map afterwards.
+    // ROWS_THREADS / 64 waves share one vertex: each takes every nwv-th group
+    // of neighbour lists and every nwv-th block of row words
     extern __shared__ unsigned long long sh[];
+    __shared__ int s_cnt;
     const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int nw = BYTES ? (n + 7) / 8 : (n + 63) / 64;  // 64-bit words of the map
     unsigned long long* vis = sh;
     unsigned char* vis8 = (unsigned char*)sh;
@@ -123,12 +130,13 @@ __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__
     constexpr int G = 8;  // neighbours u whose lists are walked together
     for (int a = blockIdx.x; a < R; a += gridDim.x) {
         const uint32_t v = orig[a];
-        for (int w = lane; w < nw; w += 64) vis[w] = 0;
-        for (int w = lane; w < rw; w += 64) r1[w] = 0;
+        for (int w = tid; w < nw; w += nt) vis[w] = 0;
+        for (int w = tid; w < rw; w += nt) r1[w] = 0;
+        if (tid == 0) s_cnt = 0;
         __syncthreads();
-        if (lane == 0) mark(v);
+        if (tid == 0) mark(v);
         const uint64_t b0 = off[v], b1 = off[v + 1];
-        for (uint64_t j = b0 + lane; j < b1; j += 64) {
+        for (uint64_t j = b0 + tid; j < b1; j += nt) {
             const uint32_t u = adj[j];
             mark(u);
             const int la = loc[u];
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__
         }
         // N(u) for G neighbours u at a time as one flat range of entries:
         // lane i < G holds u_i's list start and the prefix of list lengths
-        for (uint64_t j0 = b0; j0 < b1; j0 += G) {
+        for (uint64_t j0 = b0 + (uint64_t)wv * G; j0 < b1; j0 += (uint64_t)nwv * G) {
             uint64_t st = 0, len = 0;
             if (lane < G && j0 + lane < b1) {
                 const uint32_t u = adj[j0 + lane];
@@ -173,13 +181,13 @@ __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__
         }
         __syncthreads();
         int cnt = 0;
-        for (int w = lane; w < nw; w += 64) cnt += __popcll(vis[w]);  // bytes are 0 or 1
+        for (int w = tid; w < nw; w += nt) cnt += __popcll(vis[w]);  // bytes are 0 or 1
         for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
-        if (lane == 0) n2out[a] = cnt - 1;  // v itself is in the bitmap
-        for (int w = lane; w < rw; w += 64) A1[(size_t)a * rw + w] = r1[w];
+        if (lane == 0) atomicAdd(&s_cnt, cnt);
+        for (int w = tid; w < rw; w += nt) A1[(size_t)a * rw + w] = r1[w];
         // A2 row: bit b iff orig[b] was reached (b != a), one ballot per
         // word; eight words per step keep eight orig loads in flight
-        for (int w0 = 0; w0 < rw; w0 += 8) {
+        for (int w0 = wv * 8; w0 < rw; w0 += nwv * 8) {
             uint32_t xs[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -194,6 +202,7 @@ __global__ __launch_bounds__(64) void root_rows_lds(const uint64_t* __restrict__
             }
         }
         __syncthreads();
+        if (tid == 0) n2out[a] = s_cnt - 1;  // v itself is in the map
     }
 }
 
@@ -426,7 +435,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
         int32_t* n2p = d_n2.as<int32_t>();
         void* rargs[] = {(void*)&offp, (void*)&adjp, (void*)&locp, (void*)&origp, (void*)&Rv,
                          (void*)&rwv,  (void*)&nv,   (void*)&A1p,  (void*)&A2p,   (void*)&n2p};
-        QC_HIP(hipLaunchKernel(rows_kernel, dim3(lblocks), dim3(64), rargs, lds_bytes, s));
+        QC_HIP(hipLaunchKernel(rows_kernel, dim3(lblocks), dim3(qcdev::ROWS_THREADS), rargs, lds_bytes, s));
     } else {
         if (!g->has_n2) {
             if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;

@@ -56,3 +56,26 @@ def test_schedule_changing_the_result_raises(monkeypatch):
         assert "changed the mining result" in str(e)
     else:
         raise AssertionError("expected RuntimeError")
+
+
+def test_lucky_single_timing_does_not_leave_the_defaults(monkeypatch):
+    # split_chunk 64 times fast once, then slower than the defaults: the
+    # confirmation runs keep the defaults
+    seen = {"n": 0}
+
+    def fake_mine(g, ga, ms, sched, mode, window, **kw):
+        if kw["split_chunk"] == 64:
+            seen["n"] += 1
+            return _Res(90.0 if seen["n"] == 1 else 110.0)
+        return _Res(100.0)
+
+    import cuqc_amd
+    monkeypatch.setattr(cuqc_amd, "mine", fake_mine)
+
+    class G:
+        def checksum(self):
+            return 1
+
+    s = at.tune([G()], [0.9], [10], reps=1)
+    assert s.split_chunk == at.DEFAULT["split_chunk"]
+    assert s.kernel_ms == 100.0
