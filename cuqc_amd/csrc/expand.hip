@@ -1877,16 +1877,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                         // reads first (indegree, exdegree, label, then id,
                         // matrix index, lvl2adj)
                         // (child positions stay below t: a stride of t rounded
-                        // up to 64 entries suffices)
+                        // up to 64 entries suffices).  When the children lie
+                        // past the depth window their lvl2adj is never kept,
+                        // and the parent's exdegree, updated by every branch,
+                        // takes that place.
                         const uint32_t Tt = (ref.t + 63u) & ~63u;
                         const uint32_t k = (uint32_t)a.lds_masks / (Tt * 4);
+                        const bool keys = !(c.g.max_depth >= 0 && (int)ref.depth + 1 > c.g.max_depth);
                         int* lp = (int*)lds;
                         if (k >= 1) c.w.cin = lp;
                         if (k >= 2) c.w.cex = lp + Tt;
                         if (k >= 3) c.w.clab = lp + 2 * Tt;
                         if (k >= 4) c.w.cid = lp + 3 * Tt;
                         if (k >= 5) c.w.cpp = lp + 4 * Tt;
-                        if (k >= 6) c.w.cl2 = lp + 5 * Tt;
+                        if (k >= 6) {
+                            if (keys)
+                                c.w.cl2 = lp + 5 * Tt;
+                            else
+                                c.w.pex = lp + 5 * Tt;
+                        }
                     }
                     c.cap = a.slab_tcap;
                 }
