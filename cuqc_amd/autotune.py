@@ -36,14 +36,14 @@ import time
 from dataclasses import asdict, dataclass, field
 
 SPACE: dict[str, list[int]] = {
-    "split_chunk": [8, 16, 32, 64, -1],
+    "split_chunk": [16, 32, 64, 128, -1],
     "split_min": [1, 512, 1024],
     "mid_tile": [0, 512, 1024],
     "lds_wide": [-1, 256, 512, 1024],
     "lds_tile": [64, 128, 256],
     "workgroups": [1024, 2048, 3072, 0],
 }
-DEFAULT = {"lds_tile": 128, "split_chunk": 32, "split_min": 512, "mid_tile": 0, "lds_wide": 0, "workgroups": 0}
+DEFAULT = {"lds_tile": 128, "split_chunk": 64, "split_min": 512, "mid_tile": 0, "lds_wide": 0, "workgroups": 0}
 ORDER = ("split_chunk", "lds_wide", "split_min", "mid_tile", "lds_tile", "workgroups")
 
 _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
@@ -52,7 +52,7 @@ _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
 @dataclass
 class Schedule:
     lds_tile: int = 128
-    split_chunk: int = 32
+    split_chunk: int = 64
     split_min: int = 512
     mid_tile: int = 0
     lds_wide: int = 0

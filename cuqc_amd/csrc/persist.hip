@@ -293,7 +293,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.stats = ws.stats.as<DevStats>();
         A.error = ws.err.as<int>();
         A.queue_mode = 1;
-        A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 32 : p.split_chunk);
+        A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 64 : p.split_chunk);
         A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : 512);
         A.q_refs = ws.q.as<TaskRef>();
         A.q_ready = ws.ready.as<unsigned int>();

@@ -59,7 +59,7 @@ typedef struct qc_params {
     int32_t workgroups;      /* 0 = auto (persistent: CUs x occupancy) */
     int32_t lds_tile;        /* largest task (vertices) staged in LDS; 0 = auto */
     int32_t split_chunk;     /* persistent mode: branches per work item when a task above
-                                split_min is split across waves; 0 = 32, < 0 = never split */
+                                split_min is split across waves; 0 = 64, < 0 = never split */
     int32_t max_depth;       /* search window: expand tasks of depth <= max_depth; <= 0: all */
     uint64_t task_capacity;  /* vertex slots per task pool; 0 = auto */
     uint64_t clique_capacity;/* raw-clique vertex slots; 0 = auto */

@@ -59,12 +59,12 @@ def test_schedule_changing_the_result_raises(monkeypatch):
 
 
 def test_lucky_single_timing_does_not_leave_the_defaults(monkeypatch):
-    # split_chunk 64 times fast once, then slower than the defaults: the
+    # split_chunk 128 times fast once, then slower than the defaults: the
     # confirmation runs keep the defaults
     seen = {"n": 0}
 
     def fake_mine(g, ga, ms, sched, mode, window, **kw):
-        if kw["split_chunk"] == 64:
+        if kw["split_chunk"] == 128:
             seen["n"] += 1
             return _Res(90.0 if seen["n"] == 1 else 110.0)
         return _Res(100.0)
