@@ -34,10 +34,10 @@ HBM_PEAK_GBS = 8000.0
 # headline form, from one rocprofv3 FETCH_SIZE and one WRITE_SIZE pass, each
 # scaled by the counter's reading for a 256 MiB device copy (FETCH x2.0,
 # WRITE x1.0; profiles/hbm_traffic_coexpr8k_depth1.md).  Measured for the
-# schedule lds_tile 128 / split_chunk 64 / split_min 1024 / lds_wide 512 /
-# workgroups 1024;
+# library-default schedule (lds_tile 128 / split_chunk 64 / split_min 512 /
+# lds_wide 512);
 # None where not measured.
-TRAFFIC: dict = {"coexpr8k": 6909708685}
+TRAFFIC: dict = {"coexpr8k": 7076719273}
 
 
 def _log(msg: str) -> None:
