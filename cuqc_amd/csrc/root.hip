@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -325,7 +326,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
 
     // ---- filter (main.cu:872-885) ----
     std::vector<int32_t> lab(n), exd(n);
-    std::vector<int32_t> rem, rmv;
+    std::vector<int32_t> rem;
     std::vector<int> mark(n, -1);
     for (int i = 0; i < n; i++) {
         int deg = (int)(g->off[i + 1] - g->off[i]);
@@ -334,35 +335,48 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
         if (pass && deg < ms - 1) pass = n2_capped(g, i, mark, ms - 1) >= ms - 1;
         if (pass && g->has_n2) pass = (long)(g->off2[i + 1] - g->off2[i]) >= ms - 1;
         lab[i] = pass ? 0 : -1;
-        (pass ? rem : rmv).push_back(i);
+        if (pass) rem.push_back(i);
     }
     // ---- peel (main.cu:889-949) ----
-    int nc = n;
-    while ((int)rem.size() < nc / 2) {
-        nc = (int)rem.size();
-        for (int v : rem) exd[v] = 0;
-        for (int v : rem)
-            for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++)
-                if (lab[g->adj[j]] == 0) exd[g->adj[j]]++;
-        std::vector<int32_t> keep;
-        rmv.clear();
-        for (int v : rem) {
-            if (exd[v] >= thr)
-                keep.push_back(v);
-            else {
-                lab[v] = -1;
-                rmv.push_back(v);
+    // The reference's rounds and queue compute the core in which every
+    // vertex has at least thr neighbours inside it (unique, whatever the
+    // removal order), with exd = the degree inside it.  Here: degrees inside
+    // the filtered set (host threads over the vertices), then one queue that
+    // only walks the lists of removed filtered vertices.
+    {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const size_t nrem = rem.size();
+        const unsigned nth = nrem > 65536 ? hw : 1u;
+        auto count = [&](size_t lo, size_t hi) {
+            for (size_t k = lo; k < hi; k++) {
+                const int v = rem[k];
+                int d = 0;
+                for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) d += lab[g->adj[j]] == 0;
+                exd[v] = d;
             }
+        };
+        if (nth == 1) {
+            count(0, nrem);
+        } else {
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nth; t++)
+                th.emplace_back(count, nrem * t / nth, nrem * (t + 1) / nth);
+            for (auto& x : th) x.join();
         }
-        rem.swap(keep);
     }
-    for (size_t k = 0; k < rmv.size(); k++) {
-        int v = rmv[k];
+    std::vector<int32_t> queue;
+    for (int v : rem)
+        if (exd[v] < thr) {
+            lab[v] = -1;
+            queue.push_back(v);
+        }
+    for (size_t k = 0; k < queue.size(); k++) {
+        const int v = queue[k];
         for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) {
-            uint32_t u = g->adj[j];
+            const uint32_t u = g->adj[j];
             if (lab[u] == 0 && --exd[u] < thr) {
                 lab[u] = -1;
-                rmv.push_back(u);
+                queue.push_back(u);
             }
         }
     }
