@@ -1444,6 +1444,29 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // the items: all slots claimed at once, refs written by the lanes,
         // one release fence, then the ready flags (relaxed stores)
         const int n_items = (hi - bbeg + ch - 1) / ch;
+        // item boundaries balance the work rather than the branch count:
+        // branch b builds a child from about t0 - b vertices, so with
+        // S(b) = sum of (t0 - x) over x in [bbeg, b), boundary j is the first
+        // b with S(b) >= j S(hi) / n_items (kept strictly increasing)
+        const long long t0l = t0;
+        auto S = [&](long long b) {
+            return (b - bbeg) * t0l - (b * (b - 1) - (long long)bbeg * (bbeg - 1)) / 2;
+        };
+        const long long s_tot = S(hi);
+        auto boundary = [&](int j) -> int {
+            if (j <= 0) return bbeg;
+            if (j >= n_items) return hi;
+            const long long target = s_tot * j / n_items;
+            int lo = bbeg, up = hi;
+            while (lo < up) {
+                const int mid = (lo + up) >> 1;
+                if (S(mid) >= target)
+                    up = mid;
+                else
+                    lo = mid + 1;
+            }
+            return min(max(lo, bbeg + j), hi - (n_items - j));
+        };
         const int top = a.n_buckets - 1;
         unsigned long long ri0 = 0;
         if (c.lane == 0) {
@@ -1459,7 +1482,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             const unsigned long long ri = ri0 + (unsigned long long)j;
             if (ri >= a.q_cap) continue;
             const unsigned long long gi = ri + (unsigned long long)top * a.q_cap;
-            const int b0 = bbeg + j * ch;
+            const int b0 = boundary(j), b1 = boundary(j + 1);
             TaskRef r;
             r.off = slot;
             r.t = (uint32_t)ts;
@@ -1467,7 +1490,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             r.inst = ref.inst;
             r.bound = (uint64_t)c.slot;  // the task the item belongs to
             r.b0 = (uint32_t)(b0 - pre);
-            r.b1 = (uint32_t)(min(b0 + ch, hi) - pre);
+            r.b1 = (uint32_t)(b1 - pre);
             if (a.trace_parent) a.trace_parent[gi] = -2;
             a.q_refs[gi] = r;
         }
