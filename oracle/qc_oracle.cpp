@@ -43,6 +43,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -102,9 +106,15 @@ struct Oracle {
     const int* adj = nullptr;
     const uint64_t* off2 = nullptr;  // optional full N2 lists
     const int* adj2 = nullptr;
-    // restricted lists (root vertices only)
-    std::vector<uint64_t> roff, r2off;
-    std::vector<int> radj, r2adj;
+    // restricted lists (root vertices only): storage, and the pointers every
+    // step reads (worker copies of a multi-threaded run share the storage of
+    // the oracle that built it)
+    std::vector<uint64_t> roff_s, r2off_s;
+    std::vector<int> radj_s, r2adj_s;
+    const uint64_t* roff = nullptr;
+    const uint64_t* r2off = nullptr;
+    const int* radj = nullptr;
+    const int* r2adj = nullptr;
     std::vector<char> in_root;
 
     double gamma = 0.9;
@@ -630,15 +640,15 @@ struct Oracle {
     void build_restricted(const std::vector<Vtx>& root) {
         in_root.assign(n, 0);
         for (auto& x : root) in_root[x.id] = 1;
-        roff.assign(n + 1, 0);
-        r2off.assign(n + 1, 0);
-        radj.clear();
-        r2adj.clear();
+        roff_s.assign(n + 1, 0);
+        r2off_s.assign(n + 1, 0);
+        radj_s.clear();
+        r2adj_s.clear();
         std::vector<int> mark(n, -1), buf;
         for (int v = 0; v < n; v++) {
             if (in_root[v]) {
                 for (uint64_t j = off[v]; j < off[v + 1]; j++)
-                    if (in_root[adj[j]]) radj.push_back(adj[j]);
+                    if (in_root[adj[j]]) radj_s.push_back(adj[j]);
                 buf.clear();
                 if (off2) {
                     for (uint64_t j = off2[v]; j < off2[v + 1]; j++)
@@ -661,11 +671,15 @@ struct Oracle {
                     }
                 }
                 std::sort(buf.begin(), buf.end());
-                r2adj.insert(r2adj.end(), buf.begin(), buf.end());
+                r2adj_s.insert(r2adj_s.end(), buf.begin(), buf.end());
             }
-            roff[v + 1] = radj.size();
-            r2off[v + 1] = r2adj.size();
+            roff_s[v + 1] = radj_s.size();
+            r2off_s[v + 1] = r2adj_s.size();
         }
+        roff = roff_s.data();
+        r2off = r2off_s.data();
+        radj = radj_s.data();
+        r2adj = r2adj_s.data();
     }
 
     // Task features for the cost model (12 values), computed from a record
@@ -766,6 +780,119 @@ struct Oracle {
         }
         return 0;
     }
+
+    // The same search on `threads` host threads (clean map mode, no trace):
+    // tasks are independent once the map starts clean, so workers take task
+    // records from a shared pool, run their subtrees depth first and hand
+    // records back to the pool whenever it runs low.  The counters are sums
+    // over tasks and the raw sets a multiset, so both equal the sequential
+    // run's; only the order of the raw sets differs.
+    int run_mt(int threads) {
+        std::vector<Vtx> root = build_root();
+        st.root_size = root.size();
+        if (root.empty()) return 0;
+        build_restricted(root);
+        struct Item {
+            std::vector<Vtx> rec;
+            int depth;
+        };
+        std::mutex mu;
+        std::condition_variable cv;
+        std::deque<Item> pool;
+        int busy = 0;
+        pool.push_back(Item{std::move(root), 0});
+        std::vector<Oracle> ws(threads);
+        for (Oracle& w : ws) {
+            w.n = n;
+            w.off = off;
+            w.adj = adj;
+            w.off2 = off2;
+            w.adj2 = adj2;
+            w.roff = roff;
+            w.r2off = r2off;
+            w.radj = radj;
+            w.r2adj = r2adj;
+            w.gamma = gamma;
+            w.ms = ms;
+            w.mindeg = mindeg;
+            w.rb0 = rb0;
+            w.rb1 = rb1;
+            w.max_depth = max_depth;
+            w.pos.assign(n, -1);
+            w.remaining.assign(n + 1, 0);
+            w.removed.assign(n + 1, 0);
+            w.cind.assign(n + 1, 0);
+        }
+        auto work = [&](Oracle& w) {
+            std::vector<Item> stack, kids;
+            for (;;) {
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return !pool.empty() || busy == 0; });
+                    if (pool.empty()) return;
+                    stack.push_back(std::move(pool.front()));
+                    pool.pop_front();
+                    busy++;
+                }
+                while (!stack.empty()) {
+                    Item it = std::move(stack.back());
+                    stack.pop_back();
+                    if ((uint64_t)it.depth > w.st.max_depth) w.st.max_depth = it.depth;
+                    if (max_depth >= 0 && it.depth > max_depth) {
+                        w.st.beyond++;
+                        continue;
+                    }
+                    w.cur_depth = it.depth;
+                    kids.clear();
+                    const int d = it.depth + 1;
+                    w.expand_task(it.rec, [&](std::vector<Vtx>&& c, int) { kids.push_back(Item{std::move(c), d}); });
+                    for (auto k = kids.rbegin(); k != kids.rend(); ++k) stack.push_back(std::move(*k));
+                    // share the oldest (shallowest) records while others wait
+                    if (stack.size() > 1) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        size_t give = 0;
+                        while (pool.size() < (size_t)threads && stack.size() > 1 && give < stack.size() / 2) {
+                            pool.push_back(std::move(stack.front()));
+                            stack.erase(stack.begin());
+                            give++;
+                        }
+                        if (give) cv.notify_all();
+                    }
+                }
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    busy--;
+                    if (busy == 0 && pool.empty()) cv.notify_all();
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (int i = 1; i < threads; i++) th.emplace_back(work, std::ref(ws[i]));
+        work(ws[0]);
+        for (auto& t : th) t.join();
+        for (Oracle& w : ws) {
+            st.tasks += w.st.tasks;
+            st.branches += w.st.branches;
+            st.children += w.st.children;
+            st.raw_cliques += w.st.raw_cliques;
+            st.sum_t += w.st.sum_t;
+            st.sum_t_written += w.st.sum_t_written;
+            st.sum_clique_len += w.st.sum_clique_len;
+            st.w_adj += w.st.w_adj;
+            st.lookahead_hits += w.st.lookahead_hits;
+            st.stale_oob += w.st.stale_oob;
+            st.stale_hits += w.st.stale_hits;
+            st.max_t = std::max(st.max_t, w.st.max_t);
+            st.max_depth = std::max(st.max_depth, w.st.max_depth);
+            st.beyond += w.st.beyond;
+            st.root_branches += w.st.root_branches;
+            for (size_t i = 0; i + 1 < w.clq_off.size(); i++) {
+                clq_v.insert(clq_v.end(), w.clq_v.begin() + w.clq_off[i], w.clq_v.begin() + w.clq_off[i + 1]);
+                clq_off.push_back(clq_v.size());
+            }
+        }
+        return 0;
+    }
 };
 
 }  // namespace
@@ -792,6 +919,9 @@ struct orc_result {
 // Mine a graph to completion.  off2/adj2 may be NULL (N2 derived from N).
 orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                             double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth);
+orc_result* orc_mine_window_mt(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
+                               double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth,
+                               int threads);
 
 orc_result* orc_mine(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                      double gamma, int min_size, int faithful, int trace) {
@@ -802,6 +932,14 @@ orc_result* orc_mine(int n, const uint64_t* off, const int* adj, const uint64_t*
 // last -rb0 branches) and to tasks of depth <= max_depth (max_depth < 0: all).
 orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                             double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth) {
+    return orc_mine_window_mt(n, off, adj, off2, adj2, gamma, min_size, faithful, trace, rb0, rb1, max_depth, 1);
+}
+
+// threads > 1: the multi-threaded search (clean map mode and no trace only)
+orc_result* orc_mine_window_mt(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
+                               double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth,
+                               int threads) {
+    if (threads > 1 && (faithful || trace)) return nullptr;
     Oracle o;
     o.n = n;
     o.off = off;
@@ -816,7 +954,10 @@ orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const ui
     o.rb1 = rb1;
     o.max_depth = max_depth;
     o.build_tables();
-    o.run();
+    if (threads > 1)
+        o.run_mt(threads);
+    else
+        o.run();
     orc_result* r = new orc_result;
     r->off = std::move(o.clq_off);
     r->v = std::move(o.clq_v);
