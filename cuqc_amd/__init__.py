@@ -56,7 +56,7 @@ _STAT_U64 = ["tasks", "branches", "children", "raw_cliques", "max_cliques", "sum
 
 
 class _Stats(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("reserved", C.c_uint64 * 2)] + [
+    _fields_ = [(n, C.c_uint64) for n in _STAT_U64] + [("pool_peak", C.c_uint64), ("reserved", C.c_uint64)] + [
         ("t_init_ms", C.c_double), ("t_enum_ms", C.c_double), ("t_nonmax_ms", C.c_double), ("t_kernel_ms", C.c_double)]
 
 
@@ -270,8 +270,8 @@ def _result_from(h: C.c_void_p, rc: int) -> Result:
         st = _Stats()
         _check(L.qc_result_stats(h, C.byref(st)))
         stats = {n: int(getattr(st, n)) for n in _STAT_U64}
-        stats.update(t_init_ms=st.t_init_ms, t_enum_ms=st.t_enum_ms, t_nonmax_ms=st.t_nonmax_ms,
-                     t_kernel_ms=st.t_kernel_ms)
+        stats.update(pool_peak=int(st.pool_peak), t_init_ms=st.t_init_ms, t_enum_ms=st.t_enum_ms,
+                     t_nonmax_ms=st.t_nonmax_ms, t_kernel_ms=st.t_kernel_ms)
         rows = C.c_uint64()
         fp, pp, sp = C.POINTER(C.c_float)(), C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)()
         _check(L.qc_result_trace(h, C.byref(rows), C.byref(fp), C.byref(pp), C.byref(sp)))

@@ -26,6 +26,13 @@ BlockCache& block_cache() {
     static BlockCache* c = new BlockCache;  // lives until the process ends
     return *c;
 }
+void trim_locked(BlockCache& bc, int dev) {
+    for (auto& kv : bc.free)
+        if (kv.first.first == dev) {
+            for (void* p : kv.second) (void)hipFree(p);
+            kv.second.clear();
+        }
+}
 }  // namespace
 
 void* pool_get(size_t cls, int dev, hipError_t* err) {
@@ -45,12 +52,7 @@ void* pool_get(size_t cls, int dev, hipError_t* err) {
     if (e != hipSuccess) {
         // give the cached blocks of this device back and try once more
         (void)hipGetLastError();
-        std::lock_guard<std::mutex> lock(bc.mu);
-        for (auto& kv : bc.free)
-            if (kv.first.first == dev) {
-                for (void* q : kv.second) (void)hipFree(q);
-                kv.second.clear();
-            }
+        pool_trim(dev);
         p = nullptr;
         e = hipMalloc(&p, cls);
     }
@@ -58,10 +60,28 @@ void* pool_get(size_t cls, int dev, hipError_t* err) {
     return e == hipSuccess ? p : nullptr;
 }
 
-void pool_put(void* p, size_t cls, int dev) {
+void pool_put(void* p, size_t cls, int dev, hipStream_t s) {
+    // the block is free once the work its stream enqueued on it has ended
+    // (usually already the case: owners release after reading results back)
+    if (s) (void)hipStreamSynchronize(s);
     BlockCache& bc = block_cache();
     std::lock_guard<std::mutex> lock(bc.mu);
     bc.free[{dev, cls}].push_back(p);
+}
+
+void pool_trim(int dev) {
+    BlockCache& bc = block_cache();
+    std::lock_guard<std::mutex> lock(bc.mu);
+    trim_locked(bc, dev);
+}
+
+size_t pool_cached_bytes(int dev) {
+    BlockCache& bc = block_cache();
+    std::lock_guard<std::mutex> lock(bc.mu);
+    size_t b = 0;
+    for (auto& kv : bc.free)
+        if (kv.first.first == dev) b += kv.first.second * kv.second.size();
+    return b;
 }
 
 int load_serialized(const char* path, qc_graph* g);
@@ -186,7 +206,7 @@ int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0,
     r->trace_feat = eo.trace_feat;
     r->trace_parent = eo.trace_parent;
     r->trace_subtree = eo.trace_subtree;
-    double t_enum_end = now_ms();
+    const double t_enum_end = eo.t_enum_end_ms > 0 ? eo.t_enum_end_ms : now_ms();
     r->st.t_init_ms = t_init - t0;
     r->st.t_enum_ms = t_enum_end - t0;
     r->st.t_kernel_ms = eo.kernel_ms;
@@ -202,10 +222,17 @@ int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0,
     r->st.w_adj = eo.ds.w_adj;
     r->st.root_branches = I.root_branches;
     r->st.levels = eo.levels;
+    r->st.pool_peak = eo.pool_peak;
     r->st.root_size = (uint64_t)I.R;
-    double tn = now_ms();
-    if ((rc = remove_nonmax_device(r->raw_off, r->raw_v, r->max_off, r->max_v, s))) return rc;
-    r->st.t_nonmax_ms = now_ms() - tn;
+    if (eo.have_max) {
+        r->max_off = eo.max_off;
+        r->max_v = eo.max_v;
+        r->st.t_nonmax_ms = eo.nonmax_ms;
+    } else {
+        const double tn = now_ms();
+        if ((rc = remove_nonmax_device(r->raw_off, r->raw_v, r->max_off, r->max_v, s))) return rc;
+        r->st.t_nonmax_ms = now_ms() - tn;
+    }
     r->st.max_cliques = r->max_off.size() - 1;
     return r->st.max_cliques == 0 ? QC_EEMPTY : QC_OK;
 }

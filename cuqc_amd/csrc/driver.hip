@@ -54,6 +54,11 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     // ---- capacities ----
     size_t freeb = 0, totalb = 0;
     QC_HIP(hipMemGetInfo(&freeb, &totalb));
+    {
+        int d = 0;
+        QC_HIP(hipGetDevice(&d));
+        freeb += pool_cached_bytes(d);
+    }
     // Buffers start near what the root needs and grow on demand up to a
     // budget; task_capacity / clique_capacity pin them instead.
     const size_t budget = std::min<size_t>(freeb / 2, (size_t)96 << 30);
@@ -252,8 +257,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
                 if ((rc = nb.alloc(ncap * sizeof(Vrec)))) return rc;
                 if (D.top) QC_HIP(hipMemcpyAsync(nb.p, D.verts.p, D.top * sizeof(Vrec), hipMemcpyDeviceToDevice, s));
                 QC_HIP(hipStreamSynchronize(s));
-                std::swap(nb.p, D.verts.p);
-                std::swap(nb.bytes, D.verts.bytes);
+                D.verts.swap(nb);
                 D.cap = ncap;
             }
             if (D.top + oc[j].slots > D.cap) {

@@ -241,6 +241,11 @@ struct Ctx {
     bool phase_on;                    // phase counters requested
     bool keys;                        // the task's children are written (inside the depth
                                       // window): only then is the child's lvl2adj kept
+    // persistent mode: this wave's private stack (entries [pbot, ptop) of a
+    // ring of priv_cap refs; uniform counters, lane 0 touches the entries)
+    TaskRef* pstack;
+    unsigned char* pbkt;
+    uint32_t pbot, ptop;
 };
 
 // phase accounting: charge the cycles since the last mark to phase k
@@ -1033,6 +1038,162 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
     return uni(b);
 }
 
+// ---------------------------------------------------------------------------
+// persistent mode: vertex pool, shared rings, private stacks
+// ---------------------------------------------------------------------------
+constexpr unsigned long long FL_IDX = (1ull << 36) - 1;  // block index field; all ones = empty list
+
+// A block of the record's size class (lane 0): the class's free list first,
+// else fresh units from the bump pointer.  Returns the Vrec offset, or ~0
+// when the pool is exhausted (the bump pointer keeps counting: the host
+// reads what the launch needed).
+__device__ unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
+    uint32_t units = 0;
+    const int cl = pool_class(nvrec, units);
+    unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
+    unsigned long long h = __hip_atomic_load(hp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    while ((h & FL_IDX) != FL_IDX) {
+        const unsigned long long idx = h & FL_IDX;
+        // the next link sits in the free block's first word; if another wave
+        // takes the block meanwhile, the tag in the head changes and the
+        // exchange fails
+        const unsigned long long nxt =
+            __hip_atomic_load((unsigned long long*)(a.pool + idx * 64), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long nh = (((h >> 36) + 1ull) << 36) | (nxt & FL_IDX);
+        if (__hip_atomic_compare_exchange_strong(hp, &h, nh, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return idx * 64;
+    }
+    const unsigned long long n = (unsigned long long)units * 64;
+    const unsigned long long off = atomicAdd(a.pool_top, n);
+    return off + n <= a.pool_cap ? off : ~0ull;
+}
+
+// Gives a block back to its class's free list (lane 0, after the wave has
+// read the record: the release orders those reads before any reuse).
+__device__ void pool_free(const ExpandArgs& a, unsigned long long off, uint32_t nvrec) {
+    uint32_t units = 0;
+    const int cl = pool_class(nvrec, units);
+    unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
+    const unsigned long long idx = off / 64;
+    unsigned long long h = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        __hip_atomic_store((unsigned long long*)(a.pool + off), h & FL_IDX, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long nh = (((h >> 36) + 1ull) << 36) | idx;
+        if (__hip_atomic_compare_exchange_strong(hp, &h, nh, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+
+// Claims up to n (<= 64) consecutive free cells of bucket b for producers
+// (whole wave; lane j checks the cell of position pos + j).  Returns how
+// many were claimed (0: the ring is full), their first position in *pos.
+// With a trace the ring must not wrap (positions index the trace rows).
+__device__ int ring_claim(Ctx& c, int b, int n, unsigned long long* pos_out) {
+    const ExpandArgs& a = *c.a;
+    unsigned long long* tp = &a.q_tail[b * QSTRIDE];
+    const unsigned long long* seq = a.q_seq + (size_t)b * a.q_cap;
+    for (int tries = 0; tries < (1 << 16); tries++) {
+        unsigned long long pos = 0;
+        if (c.lane == 0) pos = __hip_atomic_load(tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pos = rdlane64(pos, 0);
+        const unsigned long long p = pos + (unsigned long long)c.lane;
+        unsigned long long sq = 0;
+        if (c.lane < n) sq = __hip_atomic_load(&seq[p % a.q_cap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t freem = ballot(c.lane < n && sq == p);
+        const uint64_t fullm = ballot(c.lane < n && sq < p);
+        int k = freem == ~0ull ? WAVE : __builtin_ctzll(~freem);
+        k = min(k, n);
+        if (a.trace_feat && pos + (unsigned long long)k > a.q_cap) k = (int)(pos < a.q_cap ? a.q_cap - pos : 0);
+        if (k == 0) {
+            // lane 0's cell still holds the entry of the previous lap: full;
+            // otherwise the tail moved under us
+            if ((fullm & 1ull) || (a.trace_feat && pos >= a.q_cap)) return 0;
+            continue;
+        }
+        int ok = 0;
+        if (c.lane == 0) {
+            unsigned long long e = pos;
+            ok = __hip_atomic_compare_exchange_strong(tp, &e, pos + (unsigned long long)k, __ATOMIC_RELAXED,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     ? 1
+                     : 0;
+        }
+        if (rdlane(ok, 0)) {
+            *pos_out = pos;
+            return k;
+        }
+    }
+    return 0;
+}
+
+// Publishes positions [pos, pos + k) of bucket b (their refs are written):
+// one release fence for the wave's record and ref stores, then the
+// sequence words (lanes < k).
+__device__ __forceinline__ void ring_publish(Ctx& c, int b, unsigned long long pos, int k) {
+    const ExpandArgs& a = *c.a;
+    wsync();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (c.lane < k) {
+        const unsigned long long p = pos + (unsigned long long)c.lane;
+        __hip_atomic_store(&a.q_seq[(size_t)b * a.q_cap + p % a.q_cap], p + 1ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One task ref into a ring (shared = true first) or this wave's private
+// stack, counted in `pending` before anyone can take it.  Returns false
+// when both are full (the launch then reports a capacity overflow).
+__device__ bool push_ref(Ctx& c, const TaskRef& r, int bkt, bool shared, const float* feat) {
+    const ExpandArgs& a = *c.a;
+    for (int pass = 0; pass < 2; pass++) {
+        const bool ring = (pass == 0) == (shared || !a.use_priv);
+        if (ring) {
+            unsigned long long pos = 0;
+            if (ring_claim(c, bkt, 1, &pos) == 1) {
+                const unsigned long long gi = (unsigned long long)bkt * a.q_cap + pos % a.q_cap;
+                if (c.lane == 0) {
+                    __hip_atomic_fetch_add((unsigned long long*)a.pending, 1ull, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    a.q_refs[gi] = r;
+                    if (a.trace_parent) a.trace_parent[gi] = r.b1 > 0 ? -2 : c.slot;
+                    if (a.trace_feat && feat)
+                        for (int k = 0; k < 12; k++) a.trace_feat[gi * 12 + k] = feat[k];
+                }
+                ring_publish(c, bkt, pos, 1);
+                return true;
+            }
+        } else if (a.use_priv && c.ptop - c.pbot < a.priv_cap) {
+            if (c.lane == 0) {
+                __hip_atomic_fetch_add((unsigned long long*)a.pending, 1ull, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                c.pstack[c.ptop % a.priv_cap] = r;
+                c.pbkt[c.ptop % a.priv_cap] = (unsigned char)bkt;
+            }
+            c.ptop++;
+            return true;
+        }
+    }
+    if (c.lane == 0) {
+        atomicAdd(a.overflow, 1ull);
+        atomicOr(a.error, 1);
+    }
+    return false;
+}
+
+// TaskRef as eight dwords, broadcast from lane l
+__device__ __forceinline__ TaskRef bcast_ref(const TaskRef& mine, int l) {
+    uint32_t w[8];
+    __builtin_memcpy(w, &mine, sizeof(w));
+#pragma unroll
+    for (int k = 0; k < 8; k++) w[k] = (uint32_t)__builtin_amdgcn_readlane((int)w[k], l);
+    TaskRef r;
+    __builtin_memcpy(&r, w, sizeof(r));
+    return r;
+}
+
 __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int mem, int gap) {
     const ExpandArgs& a = *c.a;
     WS& w = c.w;
@@ -1058,8 +1219,7 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     unsigned long long ri = 0, si = 0;
     if (c.lane == 0) {
         if (q) {
-            ri = atomicAdd(&a.q_tail[bucket * QSTRIDE], 1ull);
-            si = atomicAdd(a.pool_top, (unsigned long long)tc);
+            si = pool_alloc(a, (uint32_t)tc);
         } else {
             ri = atomicAdd(&a.out_ctr[s].items, 1ull);
             si = atomicAdd(&a.out_ctr[s].slots, (unsigned long long)tc);
@@ -1067,17 +1227,10 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     }
     ri = rdlane64(ri, 0);
     si = rdlane64(si, 0);
-    if (q ? (ri >= a.q_cap || si + tc > a.pool_cap) : (ri >= a.out_ref_cap[s] || si + tc > a.out_vert_cap[s])) {
-        if (c.lane == 0) {
-            atomicOr(a.error, 1);
-            // the slot stays unpublished: mark it done so no consumer waits
-            if (q && ri < a.q_cap)
-                __hip_atomic_store(&a.q_ready[bucket * a.q_cap + ri], 2u, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
+    if (q ? si == ~0ull : (ri >= a.out_ref_cap[s] || si + tc > a.out_vert_cap[s])) {
+        if (c.lane == 0) atomicOr(a.error, 1);
         return;
     }
-    if (q) ri += (unsigned long long)bucket * a.q_cap;  // global slot
     Vrec* dst = (q ? a.pool : a.out_verts[s]) + si;
     for (int i = c.lane; i < tc; i += WAVE) {
         uint64_t hi = w.khi[i], lo = w.klo[i];
@@ -1089,37 +1242,34 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
         v.exdeg = 0x3FFFFFFF - (int32_t)(hi & 0x3FFFFFFFull);
         dst[i] = v;
     }
-    // slot bound of this child's own children (see TaskRef)
-    unsigned long long bnd = 0;
-    for (int i = mem + c.lane; i < tc; i += WAVE) {
-        unsigned long long x = (unsigned long long)c.g.p->d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
-        bnd += x < (unsigned long long)tc ? x : (unsigned long long)tc;
-    }
-    bnd = (unsigned long long)wave_sum64((long long)bnd);
-    if (c.lane == 0) {
-        TaskRef r;
-        r.off = si;
-        r.t = (uint32_t)tc;
-        r.depth = (uint16_t)(c.depth + 1);
-        r.inst = (uint16_t)c.inst;
-        r.bound = bnd;
-        r.b0 = 0;
-        r.b1 = 0;
-        if (q) {
-            if (a.trace_parent) a.trace_parent[ri] = c.slot;
-            // count the task before it becomes visible, then publish
-            atomicAdd((unsigned long long*)a.pending, 1ull);
-            a.q_refs[ri] = r;
-        } else {
+    TaskRef r;
+    r.off = si;
+    r.t = (uint32_t)tc;
+    r.depth = (uint16_t)(c.depth + 1);
+    r.inst = (uint16_t)c.inst;
+    r.bound = 0;
+    r.b0 = 0;
+    r.b1 = 0;
+    if (q) {
+        // dynamic order: children stay on this wave's stack (depth first)
+        // unless the model predicts the largest subtrees, which go to the
+        // shared rings at once; static order: every task goes through the
+        // rings, whose positions are dealt to the waves in turn
+        const bool shared = !a.use_priv || a.static_sched || (a.n_buckets > 1 && bucket == a.n_buckets - 1);
+        if (!push_ref(c, r, bucket, shared, feat)) return;
+    } else {
+        // slot bound of this child's own children (see TaskRef)
+        unsigned long long bnd = 0;
+        for (int i = mem + c.lane; i < tc; i += WAVE) {
+            unsigned long long x =
+                (unsigned long long)c.g.p->d2r[(int32_t)(0xFFFFFFFFu - (uint32_t)w.klo[i])] + mem + 1;
+            bnd += x < (unsigned long long)tc ? x : (unsigned long long)tc;
+        }
+        bnd = (unsigned long long)wave_sum64((long long)bnd);
+        if (c.lane == 0) {
+            r.bound = bnd;
             a.out_refs[s][ri] = r;
         }
-    }
-    if (q && a.trace_feat && c.lane == 0)
-        for (int k = 0; k < 12; k++) a.trace_feat[ri * 12 + k] = feat[k];
-    if (q) {
-        // the release store publishes the whole wave's record and ref writes
-        wsync();
-        if (c.lane == 0) __hip_atomic_store(&a.q_ready[ri], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (c.lane == 0) {
         c.st->children++;
@@ -1426,10 +1576,12 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         const int pre = bbeg;
         const int ts = t0 - pre;
         bulk_remove(ts, pre);
+        // the record plus one Vrec after it whose id counts the items still
+        // to run (the last one frees the block)
         unsigned long long slot = 0;
-        if (c.lane == 0) slot = atomicAdd(a.pool_top, (unsigned long long)ts);
+        if (c.lane == 0) slot = pool_alloc(a, (uint32_t)ts + 1u);
         slot = rdlane64(slot, 0);
-        if (slot + ts > a.pool_cap) {
+        if (slot == ~0ull) {
             if (c.lane == 0) atomicOr(a.error, 1);
             return;
         }
@@ -1441,9 +1593,8 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             v.exdeg = w.pex[i];
             a.pool[slot + i] = v;
         }
-        // the items: all slots claimed at once, refs written by the lanes,
-        // one release fence, then the ready flags (relaxed stores)
         const int n_items = (hi - bbeg + ch - 1) / ch;
+        if (c.lane == 0) a.pool[slot + ts] = Vrec{n_items, 0, 0, 0};
         // item boundaries balance the work rather than the branch count:
         // branch b builds a child from about t0 - b vertices, so with
         // S(b) = sum of (t0 - x) over x in [bbeg, b), boundary j is the first
@@ -1467,41 +1618,39 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             }
             return min(max(lo, bbeg + j), hi - (n_items - j));
         };
-        const int top = a.n_buckets - 1;
-        unsigned long long ri0 = 0;
-        if (c.lane == 0) {
-            ri0 = atomicAdd(&a.q_tail[top * QSTRIDE], (unsigned long long)n_items);
-            // only slots inside the queue count as pending (the rest are
-            // an overflow: the launch is rerun with larger queues)
-            const long long n_ok = ri0 >= a.q_cap ? 0 : (long long)min<unsigned long long>(n_items, a.q_cap - ri0);
-            if (n_ok > 0) atomicAdd((unsigned long long*)a.pending, (unsigned long long)n_ok);
-            if (n_ok < n_items) atomicOr(a.error, 1);
-        }
-        ri0 = rdlane64(ri0, 0);
-        for (int j = c.lane; j < n_items; j += WAVE) {
-            const unsigned long long ri = ri0 + (unsigned long long)j;
-            if (ri >= a.q_cap) continue;
-            const unsigned long long gi = ri + (unsigned long long)top * a.q_cap;
-            const int b0 = boundary(j), b1 = boundary(j + 1);
+        auto item_ref = [&](int j) {
             TaskRef r;
             r.off = slot;
             r.t = (uint32_t)ts;
             r.depth = ref.depth;
             r.inst = ref.inst;
-            r.bound = (uint64_t)c.slot;  // the task the item belongs to
-            r.b0 = (uint32_t)(b0 - pre);
-            r.b1 = (uint32_t)(b1 - pre);
-            if (a.trace_parent) a.trace_parent[gi] = -2;
-            a.q_refs[gi] = r;
+            r.bound = (uint64_t)c.slot;  // the task the item belongs to (trace)
+            r.b0 = (uint32_t)(boundary(j) - pre);
+            r.b1 = (uint32_t)(boundary(j + 1) - pre);
+            return r;
+        };
+        // the items go to the top ring in chunks of up to 64 claimed at once
+        // (lanes write the refs, one release fence per chunk); what the ring
+        // cannot take stays on this wave's stack
+        const int top = a.n_buckets - 1;
+        int done = 0;
+        while (done < n_items) {
+            unsigned long long pos = 0;
+            const int k = ring_claim(c, top, min(n_items - done, WAVE), &pos);
+            if (k == 0) break;
+            if (c.lane == 0)
+                __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (c.lane < k) {
+                const unsigned long long gi = (unsigned long long)top * a.q_cap + (pos + c.lane) % a.q_cap;
+                if (a.trace_parent) a.trace_parent[gi] = -2;
+                a.q_refs[gi] = item_ref(done + c.lane);
+            }
+            ring_publish(c, top, pos, k);
+            done += k;
         }
-        wsync();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        for (int j = c.lane; j < n_items; j += WAVE) {
-            const unsigned long long ri = ri0 + (unsigned long long)j;
-            if (ri < a.q_cap)
-                __hip_atomic_store(&a.q_ready[ri + (unsigned long long)top * a.q_cap], 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (; done < n_items; done++)
+            if (!push_ref(c, item_ref(done), top, false, nullptr)) break;
         ph_mark(c, PH_PREPARE);
         return;
     }
@@ -1775,6 +1924,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     c.inst = 0;
     c.slot = -1;
     c.lane = lane_id();
+    c.pstack = a.priv + (size_t)blockIdx.x * a.priv_cap;
+    c.pbkt = a.priv_bkt + (size_t)blockIdx.x * a.priv_cap;
+    c.pbot = 0;
+    c.ptop = 0;
     char* const slab = a.slab + (size_t)blockIdx.x * a.slab_bytes;
     const uint32_t mid = a.mid_tcap;
     const uint32_t wide = a.lds_wide;
@@ -1784,167 +1937,254 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     c.ph = s_ph;
     if (c.lane == 0) s_st = LStats{0, 0, 0, 0, 0, 0, 0, 0, 0};
     ph_start(c);
-    constexpr int WAIT_LIMIT = 1 << 22;
-    int idle = 0;      // consecutive empty polls (back-off)
-    int sinst = -1;    // instance the wave's counters belong to
+    // consecutive idle polls before the launch is declared stuck (about a
+    // minute of back-off): every wave then leaves with error bit 4
+    constexpr int IDLE_LIMIT = 1 << 22;
+    const unsigned long long G = gridDim.x;
+    int idle = 0;       // consecutive empty polls (back-off)
+    int idle_total = 0;
+    int sinst = -1;     // instance the wave's counters belong to
+    // static order: this wave takes positions blockIdx.x, +G, +2G, ... of
+    // the single ring
+    unsigned long long spos = blockIdx.x;
+
+    // positions claimed from a ring and not yet run: lane q holds the ref
+    // of position qh + q
+    TaskRef mine;
+    int nq = 0, qi = 0;
+    unsigned long long qbase = 0, qh = 0;
     for (;;) {
-        int state = 0;  // 0 nothing published, 1 got slots, 2 done, 3 lost a race
-        unsigned long long h = 0;
-        int nk = 1;     // consecutive slots claimed
-        if (c.lane == 0) {
-            int err = __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (err & 4) {
-                state = 2;  // a slot never got published: every wave leaves
-            } else {
-                // highest non-empty bucket first
-                for (int b = a.n_buckets - 1; b >= 0 && (state == 0 || state == 3); b--) {
-                    unsigned long long* hp = &a.q_head[b * QSTRIDE];
-                    unsigned long long hb = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    unsigned long long tb =
-                        __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (hb < tb && hb < a.q_cap) {
+        if (rdlane(c.lane == 0 ? __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0, 0) & 4)
+            break;
+        TaskRef ref;
+        long long gslot;
+        if (qi < nq) {
+            ref = bcast_ref(mine, qi);
+            gslot = (long long)(qbase + (qh + (unsigned long long)qi) % a.q_cap);
+            qi++;
+        } else if (c.ptop != c.pbot) {
+            // this wave's own stack (depth first)
+            idle = 0;
+            if (!a.static_sched && c.ptop - c.pbot > 1) {
+                // the rings run low while others wait: hand over the oldest
+                // (shallowest, so largest) records, up to 4 per task
+                unsigned long long shared = 0;
+                if (c.lane == 0)
+                    for (int b = 0; b < a.n_buckets; b++) {
+                        const unsigned long long tb =
+                            __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned long long hb =
+                            __hip_atomic_load(&a.q_head[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        shared += tb > hb ? tb - hb : 0;
+                    }
+                shared = rdlane64(shared, 0);
+                for (int d = 0; d < 4 && shared + d < G / 4 && c.ptop - c.pbot > 1; d++) {
+                    int bk = 0;
+                    TaskRef r;
+                    if (c.lane == 0) {
+                        r = c.pstack[c.pbot % a.priv_cap];
+                        bk = c.pbkt[c.pbot % a.priv_cap];
+                    }
+                    r = bcast_ref(r, 0);
+                    bk = rdlane(bk, 0);
+                    unsigned long long pos = 0;
+                    if (ring_claim(c, bk, 1, &pos) != 1) break;
+                    if (c.lane == 0) a.q_refs[(unsigned long long)bk * a.q_cap + pos % a.q_cap] = r;
+                    ring_publish(c, bk, pos, 1);
+                    c.pbot++;
+                }
+            }
+            c.ptop--;
+            TaskRef r;
+            if (c.lane == 0) r = c.pstack[c.ptop % a.priv_cap];
+            ref = bcast_ref(r, 0);
+            gslot = -1;
+        } else {
+            // ---- the shared rings ----
+            int state = 0;  // 0 nothing published, 1 got positions, 2 done, 3 retry now
+            unsigned long long h = 0;
+            int nk = 1;     // consecutive positions claimed
+            int bsel = 0;   // their bucket
+            if (c.lane == 0) {
+                if (a.static_sched) {
+                    const unsigned long long sq =
+                        __hip_atomic_load(&a.q_seq[spos % a.q_cap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (sq == spos + 1ull) {
+                        state = 1;
+                        h = spos;
+                        spos += G;
+                    }
+                } else {
+                    // highest non-empty bucket first
+                    for (int b = a.n_buckets - 1; b >= 0 && (state == 0 || state == 3); b--) {
+                        unsigned long long* hp = &a.q_head[b * QSTRIDE];
+                        const unsigned long long hb = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned long long tb =
+                            __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (hb >= tb) continue;
                         // a very deep queue hands out several consecutive
-                        // slots per claim: one head update per claim limits
-                        // how fast tiny tasks can be taken.  Only very deep:
-                        // the slots of one claim run one after another, which
-                        // lengthens the critical path when they are heavy
-                        const unsigned long long avail = min(tb, a.q_cap) - hb;
-                        const unsigned long long grid = gridDim.x;
-                        const int k = avail >= 32 * grid ? 4 : avail >= 16 * grid ? 2 : 1;
-                        if (atomicCAS(hp, hb, hb + k) == hb) {
+                        // positions per claim: one head update per claim limits
+                        // how fast tiny tasks can be taken.  Only very deep: the
+                        // positions of one claim run one after another, which
+                        // lengthens the critical path when they are heavy.  Only
+                        // published positions are claimed.
+                        const unsigned long long avail = tb - hb;
+                        int k = avail >= 32 * G ? 4 : avail >= 16 * G ? 2 : 1;
+                        const unsigned long long* seq = a.q_seq + (size_t)b * a.q_cap;
+                        int pub = 0;
+                        while (pub < k && __hip_atomic_load(&seq[(hb + pub) % a.q_cap], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT) == hb + pub + 1ull)
+                            pub++;
+                        if (pub == 0) {
+                            state = 3;  // claimed by a producer, about to be published
+                            continue;
+                        }
+                        unsigned long long e = hb;
+                        if (__hip_atomic_compare_exchange_strong(hp, &e, hb + (unsigned long long)pub, __ATOMIC_RELAXED,
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                             state = 1;
-                            h = hb + (unsigned long long)b * a.q_cap;
-                            nk = k;
+                            h = hb;
+                            nk = pub;
+                            bsel = b;
                         } else {
                             state = 3;
                         }
                     }
                 }
-                // every queue looked empty: finished once nothing is pending
-                // (a task counts itself out only after pushing its children).
-                // Polls are relaxed: an acquire invalidates the CU's vector
-                // L1 under the waves still working.
+                // every ring looked empty: finished once nothing is pending (a
+                // task counts itself out only after pushing its children).
+                // Polls are relaxed: an acquire invalidates the CU's vector L1
+                // under the waves still working.
                 if (state == 0 && __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
                     state = 2;
             }
-        }
-        state = rdlane(state, 0);
-        if (state == 2) break;
-        if (state == 3) continue;
-        if (state == 0) {
-            idle = idle < 16 ? idle + 1 : 16;
-            for (int k = 0; k < idle; k++) __builtin_amdgcn_s_sleep(32);
+            state = rdlane(state, 0);
+            if (state == 2) break;
+            if (state == 3) continue;
+            if (state == 0) {
+                idle = idle < 16 ? idle + 1 : 16;
+                if (++idle_total >= IDLE_LIMIT) {
+                    if (c.lane == 0) atomicOr(a.error, 4);
+                    break;
+                }
+                for (int k = 0; k < idle; k++) __builtin_amdgcn_s_sleep(32);
+                continue;
+            }
+            idle = 0;
+            idle_total = 0;
+            h = rdlane64(h, 0);
+            nk = rdlane(nk, 0);
+            bsel = rdlane(bsel, 0);
+            // one acquire per claim (pairs with the producers' release before
+            // the sequence words), then lane q reads the ref of position h + q
+            // and gives the cell back for the next lap
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const unsigned long long base = (unsigned long long)bsel * a.q_cap;
+            if (c.lane < nk) mine = a.q_refs[base + (h + c.lane) % a.q_cap];
+            wsync();
+            if (c.lane < nk)
+                __hip_atomic_store(&a.q_seq[base + (h + c.lane) % a.q_cap], h + (unsigned long long)c.lane + a.q_cap,
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            nq = nk;
+            qi = 0;
+            qbase = base;
+            qh = h;
             continue;
         }
-        idle = 0;
-        h = rdlane64(h, 0);
-        nk = rdlane(nk, 0);
-        for (int q = 0; q < nk; q++) {
-            const unsigned long long hq = h + (unsigned long long)q;
-            unsigned int rd = 0;
-            if (c.lane == 0) {
-                // the producer published or is about to publish this slot
-                for (int it = 0; it < WAIT_LIMIT; it++) {
-                    rd = __hip_atomic_load(&a.q_ready[hq], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (rd) break;
-                    __builtin_amdgcn_s_sleep(2);
-                }
-                if (!rd) atomicOr(a.error, 4);
-            }
-            rd = (unsigned int)rdlane((int)rd, 0);
-            ph_mark(c, PH_IDLE);
-            if (rd == 1) {
-                // one acquire per task taken (pairs with the producer's release
-                // store of the ready flag)
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const TaskRef ref = a.q_refs[hq];
-                const long long tl_start = a.timeline ? wall_clock64() : 0;
-                if (a.graphs) {
-                    c.g = load_graph(&a.graphs[ref.inst]);
-                    c.inst = ref.inst;
-                }
-                if (a.inst_stats && (int)ref.inst != sinst) {
-                    if (sinst >= 0) flush_stats(c, &a.inst_stats[sinst]);
-                    sinst = ref.inst;
-                }
-                c.slot = ref.b1 > 0 ? (long long)ref.bound : (long long)hq;
-                // workspace: LDS with local matrices, LDS arrays with root
-                // matrices, slab with local matrices, or slab in root space
-                const bool fits = ref.t <= a.slab_tcap;
-                const bool local = ref.t <= (uint32_t)TL || (ref.t > wide && ref.t <= mid);
-                // (built per task: only one workspace descriptor is live)
-                if (ref.t <= (uint32_t)TL) {
-                    c.w = make_ws(lds, TL, true, (TL + 63) / 64);
-                    c.cap = TL;
-                } else if (ref.t <= wide) {
-                    c.w = make_ws_split(lds, wide,
-                                        a.lds_masks >= 0 ? (uint64_t*)(lds + a.lds_masks)
-                                                         : (uint64_t*)(slab + ws_bytes(a.slab_tcap, false, 0)),
-                                        a.slab_mwords);
-                    c.cap = wide;
-                } else if (ref.t <= mid) {
-                    c.w = make_ws(slab, mid, true, (mid + 63) / 64);
-                    if (a.lds_masks >= 0) {
-                        c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
-                        c.w.mk1 = c.w.mk0 + (mid + 63) / 64;
-                    }
-                    c.cap = mid;
-                } else {
-                    c.w = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
-                    if (a.lds_masks >= 0) {
-                        c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
-                        c.w.mk1 = c.w.mk0 + a.slab_mwords;
-                        // the LDS below the masks (the wide tier's arrays) is
-                        // free during a slab task: as many of the child's
-                        // arrays as fit go there, the ones every pruning pass
-                        // reads first (indegree, exdegree, label, then id,
-                        // matrix index, lvl2adj)
-                        // (child positions stay below t: a stride of t rounded
-                        // up to 64 entries suffices).  When the children lie
-                        // past the depth window their lvl2adj is never kept,
-                        // and the parent's exdegree, updated by every branch,
-                        // takes that place.
-                        const uint32_t Tt = (ref.t + 63u) & ~63u;
-                        const uint32_t k = (uint32_t)a.lds_masks / (Tt * 4);
-                        const bool keys = !(c.g.max_depth >= 0 && (int)ref.depth + 1 > c.g.max_depth);
-                        int* lp = (int*)lds;
-                        if (k >= 1) c.w.cin = lp;
-                        if (k >= 2) c.w.cex = lp + Tt;
-                        if (k >= 3) c.w.clab = lp + 2 * Tt;
-                        if (k >= 4) c.w.cid = lp + 3 * Tt;
-                        if (k >= 5) c.w.cpp = lp + 4 * Tt;
-                        if (k >= 6) {
-                            if (keys)
-                                c.w.cl2 = lp + 5 * Tt;
-                            else
-                                c.w.pex = lp + 5 * Tt;
-                        }
-                    }
-                    c.cap = a.slab_tcap;
-                }
-                if (!fits) {
-                    if (c.lane == 0) atomicOr(a.error, 2);
-                } else if (local) {
-                    process_task<true>(c, ref);
-                } else {
-                    process_task<false>(c, ref);
-                }
-                wsync();
-                if (a.timeline && c.lane == 0) {
-                    unsigned long long* r = a.timeline + 4 * hq;
-                    r[0] = (unsigned long long)tl_start;
-                    r[1] = (unsigned long long)wall_clock64();
-                    r[2] = ((unsigned long long)blockIdx.x << 32) | ref.t;
-                    r[3] = (unsigned long long)ref.depth | ((unsigned long long)ref.b0 << 16) |
-                           ((unsigned long long)ref.b1 << 40);
-                }
-                if (c.lane == 0)
-                    __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)-1ll, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                ph_mark(c, PH_CHILD);
-            }
-            if (rd == 0) break;  // error 4: the next poll leaves
+        ph_mark(c, PH_IDLE);
+        const long long tl_start = a.timeline ? wall_clock64() : 0;
+        if (a.graphs) {
+            c.g = load_graph(&a.graphs[ref.inst]);
+            c.inst = ref.inst;
         }
+        if (a.inst_stats && (int)ref.inst != sinst) {
+            if (sinst >= 0) flush_stats(c, &a.inst_stats[sinst]);
+            sinst = ref.inst;
+        }
+        c.slot = ref.b1 > 0 ? (long long)ref.bound : gslot;
+        // workspace: LDS with local matrices, LDS arrays with root
+        // matrices, slab with local matrices, or slab in root space
+        const bool fits = ref.t <= a.slab_tcap;
+        const bool local = ref.t <= (uint32_t)TL || (ref.t > wide && ref.t <= mid);
+        // (built per task: only one workspace descriptor is live)
+        if (ref.t <= (uint32_t)TL) {
+            c.w = make_ws(lds, TL, true, (TL + 63) / 64);
+            c.cap = TL;
+        } else if (ref.t <= wide) {
+            c.w = make_ws_split(lds, wide,
+                                a.lds_masks >= 0 ? (uint64_t*)(lds + a.lds_masks)
+                                                 : (uint64_t*)(slab + ws_bytes(a.slab_tcap, false, 0)),
+                                a.slab_mwords);
+            c.cap = wide;
+        } else if (ref.t <= mid) {
+            c.w = make_ws(slab, mid, true, (mid + 63) / 64);
+            if (a.lds_masks >= 0) {
+                c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
+                c.w.mk1 = c.w.mk0 + (mid + 63) / 64;
+            }
+            c.cap = mid;
+        } else {
+            c.w = make_ws(slab, a.slab_tcap, false, a.slab_mwords);
+            if (a.lds_masks >= 0) {
+                c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
+                c.w.mk1 = c.w.mk0 + a.slab_mwords;
+                // the LDS below the masks (the wide tier's arrays) is free
+                // during a slab task: as many of the child's arrays as fit go
+                // there, the ones every pruning pass reads first (indegree,
+                // exdegree, label, then id, matrix index, lvl2adj) (child
+                // positions stay below t: a stride of t rounded up to 64
+                // entries suffices).  When the children lie past the depth
+                // window their lvl2adj is never kept, and the parent's
+                // exdegree, updated by every branch, takes that place.
+                const uint32_t Tt = (ref.t + 63u) & ~63u;
+                const uint32_t k = (uint32_t)a.lds_masks / (Tt * 4);
+                const bool keys = !(c.g.max_depth >= 0 && (int)ref.depth + 1 > c.g.max_depth);
+                int* lp = (int*)lds;
+                if (k >= 1) c.w.cin = lp;
+                if (k >= 2) c.w.cex = lp + Tt;
+                if (k >= 3) c.w.clab = lp + 2 * Tt;
+                if (k >= 4) c.w.cid = lp + 3 * Tt;
+                if (k >= 5) c.w.cpp = lp + 4 * Tt;
+                if (k >= 6) {
+                    if (keys)
+                        c.w.cl2 = lp + 5 * Tt;
+                    else
+                        c.w.pex = lp + 5 * Tt;
+                }
+            }
+            c.cap = a.slab_tcap;
+        }
+        if (!fits) {
+            if (c.lane == 0) atomicOr(a.error, 2);
+        } else if (local) {
+            process_task<true>(c, ref);
+        } else {
+            process_task<false>(c, ref);
+        }
+        wsync();
+        if (a.timeline && gslot >= 0 && c.lane == 0) {
+            unsigned long long* r = a.timeline + 4 * (unsigned long long)gslot;
+            r[0] = (unsigned long long)tl_start;
+            r[1] = (unsigned long long)wall_clock64();
+            r[2] = ((unsigned long long)blockIdx.x << 32) | ref.t;
+            r[3] = (unsigned long long)ref.depth | ((unsigned long long)ref.b0 << 16) |
+                   ((unsigned long long)ref.b1 << 40);
+        }
+        if (c.lane == 0) {
+            // the record's block: a split item's prepared record is freed by
+            // the last of its items (count in the Vrec after the record);
+            // roots live outside the pool's blocks
+            if (ref.b1 > 0) {
+                const int left = __hip_atomic_fetch_add(&a.pool[ref.off + ref.t].id, -1, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+                if (left == 1) pool_free(a, ref.off, ref.t + 1u);
+            } else if (ref.depth > 0) {
+                pool_free(a, ref.off, ref.t);
+            }
+            __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)-1ll, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        ph_mark(c, PH_CHILD);
     }
     ph_mark(c, PH_IDLE);
     ph_flush(c);

@@ -4,31 +4,107 @@
 // marks leaves that are subsets of sets to their right and writes the
 // remaining leaves in trie DFS order.  Its output is: each distinct set that
 // is not a proper subset of another raw set, ascending, in lexicographic
-// order.  This file computes the same result with data-parallel steps:
+// order.  This file computes the same result with data-parallel steps over
+// the raw sets where the expansion kernels left them (no host round trip):
+//   0. copy every set into one contiguous array of keys; a vertex v of
+//      instance i becomes base[i] + v, so the sets of different instances of a
+//      batch share no key and are never compared with each other;
 //   1. segmented radix sort of every set (rocPRIM);
 //   2. an order-independent 64-bit hash per set, radix sort by hash, exact
 //      comparison inside equal-hash runs -> one representative per distinct
 //      set;
-//   3. an inverted index vertex -> sets (count, scan, scatter);
-//   4. one wave per set: scan the postings of its rarest vertex and test
+//   3. an inverted index key -> sets (count, scan, scatter);
+//   4. one wave per set: scan the postings of its rarest key and test
 //      "S subset of T" for every longer T by binary search;
-//   5. merge sort of the survivors in lexicographic order (rocPRIM).
+//   5. merge sort of the survivors in lexicographic key order (rocPRIM): the
+//      instances come out one after another, each in lexicographic order;
+//   6. the survivors are written out with their instance ids.
+// Every buffer is sized by upper bounds known before the first launch (sets,
+// vertices, keys), so the sequence has no host synchronization and no
+// allocation of its own: it runs in one caller-provided scratch region.
 
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "qc_internal.h"
 
 namespace qcdev {
+
+constexpr uint32_t NM_NONE = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z += 0x9e3779b97f4a7c15ULL;
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
     return z ^ (z >> 31);
+}
+
+template <class T>
+__global__ void nm_fill(T* __restrict__ p, uint64_t n, T x) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = x;
+}
+
+// lens[i] = size of raw set i (i < k), lens[k] = 0.  Caller offsets that
+// decrease or pass nv raise *err and give an empty set.
+__global__ void nm_lens(const CliqueRef* __restrict__ refs, const uint64_t* __restrict__ offs, uint64_t k,
+                        uint64_t nv, uint64_t* __restrict__ lens, uint32_t* __restrict__ err) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= k; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t len = 0;
+        if (i < k && refs) {
+            len = refs[i].len;
+        } else if (i < k) {
+            const uint64_t a = offs[i], b = offs[i + 1];
+            if (b < a || b > nv)
+                atomicOr(err, 1u);
+            else
+                len = b - a;
+        }
+        lens[i] = len;
+    }
+}
+
+// offsets past nv (overlapping caller sets) are cut to nv, raising *err:
+// every later access stays inside the nv-entry arrays
+__global__ void nm_clamp(uint64_t* __restrict__ o, uint64_t k, uint64_t nv, uint32_t* __restrict__ err) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= k; i += (uint64_t)gridDim.x * blockDim.x)
+        if (o[i] > nv) {
+            o[i] = nv;
+            atomicOr(err, 1u);
+        }
+}
+
+// set i -> keys [o[i], o[i+1]); a vertex outside its instance's range raises
+// *err and is clamped into the range (the result is then discarded)
+__global__ __launch_bounds__(64) void nm_keys(const CliqueRef* __restrict__ refs, const uint64_t* __restrict__ offs,
+                                              const uint32_t* __restrict__ verts, const uint32_t* __restrict__ base,
+                                              const uint32_t* __restrict__ limit, uint32_t nkeys, uint64_t k,
+                                              const uint64_t* __restrict__ o, uint32_t* __restrict__ keys,
+                                              uint32_t* __restrict__ inst, uint32_t* __restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    for (uint64_t s = blockIdx.x; s < k; s += gridDim.x) {
+        const uint64_t src = refs ? refs[s].off : offs[s];
+        const uint32_t in = refs ? refs[s].inst : 0u;
+        const uint32_t b = base ? base[in] : 0u;
+        const uint32_t lim = base ? limit[in] : nkeys;
+        const uint64_t dst = o[s], len = o[s + 1] - o[s];
+        bool bad = false;
+        for (uint64_t j = lane; j < len; j += 64) {
+            uint32_t v = verts[src + j];
+            if (v >= lim) {
+                bad = true;
+                v = 0;
+            }
+            keys[dst + j] = b + v;
+        }
+        if (__ballot(bad) && lane == 0) atomicOr(err, 1u);
+        if (lane == 0) inst[s] = in;
+    }
 }
 
 __global__ __launch_bounds__(64) void nm_hash(const uint64_t* __restrict__ off, const uint32_t* __restrict__ v,
@@ -71,14 +147,23 @@ __global__ void nm_dedupe(const uint64_t* __restrict__ off, const uint32_t* __re
     }
 }
 
-__global__ void nm_gather_unique(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ keep,
-                                 const uint32_t* __restrict__ pos, uint64_t k, uint32_t* __restrict__ uniq) {
+// out[pos[p]] = ids[p] for every flagged p < k
+__global__ void nm_compact(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ flag,
+                           const uint32_t* __restrict__ pos, uint64_t k, uint32_t* __restrict__ out) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < k; p += (uint64_t)gridDim.x * blockDim.x)
-        if (keep[p]) uniq[pos[p]] = ids[p];
+        if (flag[p]) out[pos[p]] = ids[p];
+}
+
+// *n = number of flagged entries (exclusive scan + last flag)
+__global__ void nm_total(const uint32_t* __restrict__ pos, const uint32_t* __restrict__ flag, uint64_t k,
+                         uint32_t* __restrict__ n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n = k ? pos[k - 1] + flag[k - 1] : 0u;
 }
 
 __global__ void nm_count(const uint64_t* __restrict__ off, const uint32_t* __restrict__ v,
-                         const uint32_t* __restrict__ uniq, uint32_t nu, uint32_t* __restrict__ cnt) {
+                         const uint32_t* __restrict__ uniq, const uint32_t* __restrict__ nu_p,
+                         uint32_t* __restrict__ cnt) {
+    const uint32_t nu = *nu_p;
     for (uint32_t u = blockIdx.x; u < nu; u += gridDim.x) {
         uint32_t s = uniq[u];
         for (uint64_t j = off[s] + threadIdx.x; j < off[s + 1]; j += blockDim.x) atomicAdd(&cnt[v[j]], 1u);
@@ -86,8 +171,10 @@ __global__ void nm_count(const uint64_t* __restrict__ off, const uint32_t* __res
 }
 
 __global__ void nm_scatter(const uint64_t* __restrict__ off, const uint32_t* __restrict__ v,
-                           const uint32_t* __restrict__ uniq, uint32_t nu, const uint32_t* __restrict__ start,
-                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ post) {
+                           const uint32_t* __restrict__ uniq, const uint32_t* __restrict__ nu_p,
+                           const uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+                           uint32_t* __restrict__ post) {
+    const uint32_t nu = *nu_p;
     for (uint32_t u = blockIdx.x; u < nu; u += gridDim.x) {
         uint32_t s = uniq[u];
         for (uint64_t j = off[s] + threadIdx.x; j < off[s + 1]; j += blockDim.x) {
@@ -98,17 +185,24 @@ __global__ void nm_scatter(const uint64_t* __restrict__ off, const uint32_t* __r
     }
 }
 
-// maximal[u] = 1 iff unique set uniq[u] is not a proper subset of another
+// maximal[u] = 1 iff unique set uniq[u] (u < nu) is not a proper subset of
+// another; 0 for nu <= u < k
 __global__ __launch_bounds__(64) void nm_subset(const uint64_t* __restrict__ off, const uint32_t* __restrict__ v,
-                                                const uint32_t* __restrict__ uniq, uint32_t nu,
-                                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ cnt,
-                                                const uint32_t* __restrict__ post, uint32_t* __restrict__ maximal) {
+                                                const uint32_t* __restrict__ uniq, const uint32_t* __restrict__ nu_p,
+                                                uint64_t k, const uint32_t* __restrict__ start,
+                                                const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ post,
+                                                uint32_t* __restrict__ maximal) {
     const int lane = threadIdx.x & 63;
-    for (uint32_t u = blockIdx.x; u < nu; u += gridDim.x) {
+    const uint32_t nu = *nu_p;
+    for (uint64_t u = blockIdx.x; u < k; u += gridDim.x) {
+        if (u >= nu) {
+            if (lane == 0) maximal[u] = 0u;
+            continue;
+        }
         const uint32_t s = uniq[u];
         const uint64_t sb = off[s], se = off[s + 1];
         const uint64_t ls = se - sb;
-        // rarest vertex of s
+        // rarest key of s
         uint32_t best = 0xFFFFFFFFu, bx = 0;
         for (uint64_t j = sb + lane; j < se; j += 64) {
             uint32_t x = v[j];
@@ -149,10 +243,19 @@ __global__ __launch_bounds__(64) void nm_subset(const uint64_t* __restrict__ off
     }
 }
 
+// entries at or past *n become NM_NONE (sorted last)
+__global__ void nm_pad(uint32_t* __restrict__ list, const uint32_t* __restrict__ n_p, uint64_t k) {
+    const uint32_t n = *n_p;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
+        if (i >= n) list[i] = NM_NONE;
+}
+
+// lexicographic order of the sets named by their index; NM_NONE last
 struct LexLess {
     const uint64_t* off;
     const uint32_t* v;
     __device__ bool operator()(uint32_t a, uint32_t b) const {
+        if (a == NM_NONE || b == NM_NONE) return a != NM_NONE && b == NM_NONE;
         uint64_t ia = off[a], ea = off[a + 1], ib = off[b], eb = off[b + 1];
         while (ia < ea && ib < eb) {
             uint32_t x = v[ia], y = v[ib];
@@ -160,9 +263,48 @@ struct LexLess {
             ia++;
             ib++;
         }
-        return (ea - ia) < (eb - ib) ? true : false;
+        return (ea - ia) < (eb - ib);
     }
 };
+
+// olens[r] = size of the r-th survivor (0 past the survivors and at k)
+__global__ void nm_out_lens(const uint64_t* __restrict__ off, const uint32_t* __restrict__ sorted,
+                            const uint32_t* __restrict__ n_p, uint64_t k, uint64_t* __restrict__ olens) {
+    const uint32_t n = *n_p;
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r <= k; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = r < n ? sorted[r] : NM_NONE;
+        olens[r] = s == NM_NONE ? 0 : off[s + 1] - off[s];
+    }
+}
+
+// survivors -> output sets (keys back to vertex ids), their instances and
+// the count (~0 when a vertex was out of range)
+__global__ __launch_bounds__(64) void nm_out_write(const uint64_t* __restrict__ off, const uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ set_inst,
+                                                   const uint32_t* __restrict__ base,
+                                                   const uint32_t* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ n_p, const uint64_t* __restrict__ oo,
+                                                   const uint32_t* __restrict__ err, uint64_t* __restrict__ out_count,
+                                                   uint64_t* __restrict__ out_off, uint32_t* __restrict__ out_v,
+                                                   uint32_t* __restrict__ out_inst) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = *n_p;
+    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const uint32_t s = sorted[r];
+        const uint32_t in = set_inst[s];
+        const uint32_t b = base ? base[in] : 0u;
+        const uint64_t src = off[s], len = off[s + 1] - off[s], dst = oo[r];
+        for (uint64_t j = lane; j < len; j += 64) out_v[dst + j] = keys[src + j] - b;
+        if (lane == 0) {
+            out_off[r] = dst;
+            if (out_inst) out_inst[r] = in;
+        }
+    }
+    if (blockIdx.x == 0 && lane == 0) {
+        out_off[n] = oo[n];
+        *out_count = *err ? ~0ull : (uint64_t)n;
+    }
+}
 
 }  // namespace qcdev
 
@@ -177,6 +319,227 @@ namespace qc {
         }                                                                       \
     } while (0)
 
+namespace {
+
+// Byte layout of the scratch region for k sets, nv vertices and nkeys keys.
+struct NmPlan {
+    size_t o, lens, oo, keys_in, keys, inst, h, hs, idx, ids, keep, pos, uniq, cnt, start, cur, post, mx, mpos, mlist,
+        sorted, ctr, temp, temp_bytes, total;
+};
+
+size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int nm_plan(uint64_t k, uint64_t nv, uint32_t nkeys, NmPlan& P, hipStream_t s) {
+    // rocPRIM temporary storage: the largest of every call below
+    size_t tb = 0, t;
+    const unsigned nvu = (unsigned)nv, ku = (unsigned)k;
+    uint32_t* u32 = nullptr;
+    uint64_t* u64 = nullptr;
+#define NM_Q(call)       \
+    do {                 \
+        t = 0;           \
+        QC_RP(call);     \
+        tb = std::max(tb, t); \
+    } while (0)
+    NM_Q(rocprim::exclusive_scan(nullptr, t, u64, u64, (uint64_t)0, (size_t)k + 1, rocprim::plus<uint64_t>(), s));
+    NM_Q(rocprim::segmented_radix_sort_keys(nullptr, t, u32, u32, nvu, ku, u64, u64 + 1, 0, 32, s));
+    NM_Q(rocprim::radix_sort_pairs(nullptr, t, u64, u64, u32, u32, (size_t)k, 0, 64, s));
+    NM_Q(rocprim::exclusive_scan(nullptr, t, u32, u32, 0u, (size_t)k, rocprim::plus<uint32_t>(), s));
+    NM_Q(rocprim::exclusive_scan(nullptr, t, u32, u32, 0u, (size_t)nkeys, rocprim::plus<uint32_t>(), s));
+    qcdev::LexLess less{nullptr, nullptr};
+    NM_Q(rocprim::merge_sort(nullptr, t, u32, u32, (size_t)k, less, s));
+#undef NM_Q
+    size_t b = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = b;
+        b += up256(std::max<size_t>(bytes, 16));
+        return at;
+    };
+    P.o = take((k + 1) * 8);
+    P.lens = take((k + 1) * 8);
+    P.oo = take((k + 1) * 8);
+    P.keys_in = take(nv * 4);
+    P.post = P.keys_in;  // keys_in is dead once the sets are sorted
+    P.keys = take(nv * 4);
+    P.inst = take(k * 4);
+    P.h = take(k * 8);
+    P.mlist = P.h;  // the hashes are dead once sorted
+    P.sorted = P.h + up256(k * 4);
+    P.hs = take(k * 8);
+    P.idx = take(k * 4);
+    P.ids = take(k * 4);
+    P.keep = take(k * 4);
+    P.pos = take(k * 4);
+    P.uniq = take(k * 4);
+    P.cnt = take((size_t)nkeys * 4);
+    P.start = take((size_t)nkeys * 4);
+    P.cur = take((size_t)nkeys * 4);
+    P.mx = take(k * 4);
+    P.mpos = take(k * 4);
+    P.ctr = take(16);
+    P.temp_bytes = tb;
+    P.temp = take(tb);
+    P.total = b;
+    return QC_OK;
+}
+
+}  // namespace
+
+size_t nonmax_scratch_bytes(uint64_t k, uint64_t nv, uint32_t nkeys, hipStream_t s) {
+    NmPlan P;
+    if (nm_plan(k, nv, nkeys, P, s)) return 0;
+    return P.total;
+}
+
+int nonmax_run(const NmInput& in, char* scr, size_t scr_bytes, const NmOutput& out, hipStream_t s) {
+    const uint64_t k = in.k, nv = in.nv;
+    if (k >= 0xFFFFFFFFull || nv >= 0xFFFFFFFFull) {
+        set_error("RemoveNonMax: too many raw sets or vertices (%llu sets, %llu vertices)", (unsigned long long)k,
+                  (unsigned long long)nv);
+        return QC_EOVERFLOW;
+    }
+    if (k == 0) {
+        hipLaunchKernelGGL(qcdev::nm_fill<uint64_t>, dim3(1), dim3(64), 0, s, out.count, (uint64_t)1, (uint64_t)0);
+        hipLaunchKernelGGL(qcdev::nm_fill<uint64_t>, dim3(1), dim3(64), 0, s, out.off, (uint64_t)1, (uint64_t)0);
+        QC_HIP(hipGetLastError());
+        return QC_OK;
+    }
+    const uint32_t nkeys = std::max<uint32_t>(in.nkeys, 1);
+    NmPlan P;
+    int rc;
+    if ((rc = nm_plan(k, nv, nkeys, P, s))) return rc;
+    if (P.total > scr_bytes) {
+        set_error("RemoveNonMax needs %zu bytes of scratch, %zu available", P.total, scr_bytes);
+        return QC_EOVERFLOW;
+    }
+    auto U32 = [&](size_t at) { return (uint32_t*)(scr + at); };
+    auto U64 = [&](size_t at) { return (uint64_t*)(scr + at); };
+    void* temp = scr + P.temp;
+    size_t tb = P.temp_bytes;
+    uint32_t* err = U32(P.ctr);
+    uint32_t* nu_p = U32(P.ctr) + 1;
+    uint32_t* nmx_p = U32(P.ctr) + 2;
+    const unsigned gk = (unsigned)std::min<uint64_t>((k + 255) / 256 + 1, 16384);
+    const unsigned gw = (unsigned)std::min<uint64_t>(k, 65536);
+    const unsigned gn = (unsigned)std::min<uint64_t>(((uint64_t)nkeys + 255) / 256, 16384);
+
+    // 0. contiguous keyed copy
+    hipLaunchKernelGGL(qcdev::nm_fill<uint32_t>, dim3(1), dim3(64), 0, s, err, (uint64_t)4, 0u);
+    hipLaunchKernelGGL(qcdev::nm_lens, dim3(gk), dim3(256), 0, s, in.refs, in.offsets, k, nv, U64(P.lens), err);
+    QC_HIP(hipGetLastError());
+    QC_RP(rocprim::exclusive_scan(temp, tb, U64(P.lens), U64(P.o), (uint64_t)0, (size_t)k + 1,
+                                  rocprim::plus<uint64_t>(), s));
+    hipLaunchKernelGGL(qcdev::nm_clamp, dim3(gk), dim3(256), 0, s, U64(P.o), k, nv, err);
+    hipLaunchKernelGGL(qcdev::nm_keys, dim3(gw), dim3(64), 0, s, in.refs, in.offsets, in.verts, in.base, in.limit,
+                       nkeys, k, U64(P.o), U32(P.keys_in), U32(P.inst), err);
+    QC_HIP(hipGetLastError());
+
+    // 1. sort every set ascending
+    tb = P.temp_bytes;
+    QC_RP(rocprim::segmented_radix_sort_keys(temp, tb, U32(P.keys_in), U32(P.keys), (unsigned int)nv,
+                                            (unsigned int)k, U64(P.o), U64(P.o) + 1, 0, 32, s));
+    const uint64_t* off = U64(P.o);
+    const uint32_t* keys = U32(P.keys);
+
+    // 2. hash, sort by hash, dedupe
+    hipLaunchKernelGGL(qcdev::nm_hash, dim3(gw), dim3(64), 0, s, off, keys, k, U64(P.h), U32(P.idx));
+    QC_HIP(hipGetLastError());
+    tb = P.temp_bytes;
+    QC_RP(rocprim::radix_sort_pairs(temp, tb, U64(P.h), U64(P.hs), U32(P.idx), U32(P.ids), (size_t)k, 0, 64, s));
+    hipLaunchKernelGGL(qcdev::nm_dedupe, dim3(gk), dim3(256), 0, s, off, keys, U64(P.hs), U32(P.ids), k,
+                       U32(P.keep));
+    QC_HIP(hipGetLastError());
+    tb = P.temp_bytes;
+    QC_RP(rocprim::exclusive_scan(temp, tb, U32(P.keep), U32(P.pos), 0u, (size_t)k, rocprim::plus<uint32_t>(), s));
+    hipLaunchKernelGGL(qcdev::nm_total, dim3(1), dim3(64), 0, s, U32(P.pos), U32(P.keep), k, nu_p);
+    hipLaunchKernelGGL(qcdev::nm_compact, dim3(gk), dim3(256), 0, s, U32(P.ids), U32(P.keep), U32(P.pos), k,
+                       U32(P.uniq));
+    QC_HIP(hipGetLastError());
+
+    // 3. inverted index
+    hipLaunchKernelGGL(qcdev::nm_fill<uint32_t>, dim3(gn), dim3(256), 0, s, U32(P.cnt), (uint64_t)nkeys, 0u);
+    hipLaunchKernelGGL(qcdev::nm_fill<uint32_t>, dim3(gn), dim3(256), 0, s, U32(P.cur), (uint64_t)nkeys, 0u);
+    hipLaunchKernelGGL(qcdev::nm_count, dim3(gw), dim3(64), 0, s, off, keys, U32(P.uniq), nu_p, U32(P.cnt));
+    QC_HIP(hipGetLastError());
+    tb = P.temp_bytes;
+    QC_RP(rocprim::exclusive_scan(temp, tb, U32(P.cnt), U32(P.start), 0u, (size_t)nkeys, rocprim::plus<uint32_t>(),
+                                  s));
+    hipLaunchKernelGGL(qcdev::nm_scatter, dim3(gw), dim3(64), 0, s, off, keys, U32(P.uniq), nu_p, U32(P.start),
+                       U32(P.cur), U32(P.post));
+    QC_HIP(hipGetLastError());
+
+    // 4. subset test
+    hipLaunchKernelGGL(qcdev::nm_subset, dim3(gw), dim3(64), 0, s, off, keys, U32(P.uniq), nu_p, k, U32(P.start),
+                       U32(P.cnt), U32(P.post), U32(P.mx));
+    QC_HIP(hipGetLastError());
+    tb = P.temp_bytes;
+    QC_RP(rocprim::exclusive_scan(temp, tb, U32(P.mx), U32(P.mpos), 0u, (size_t)k, rocprim::plus<uint32_t>(), s));
+    hipLaunchKernelGGL(qcdev::nm_total, dim3(1), dim3(64), 0, s, U32(P.mpos), U32(P.mx), k, nmx_p);
+    hipLaunchKernelGGL(qcdev::nm_compact, dim3(gk), dim3(256), 0, s, U32(P.uniq), U32(P.mx), U32(P.mpos), k,
+                       U32(P.mlist));
+    hipLaunchKernelGGL(qcdev::nm_pad, dim3(gk), dim3(256), 0, s, U32(P.mlist), nmx_p, k);
+    QC_HIP(hipGetLastError());
+
+    // 5. lexicographic order (sets past the survivors sort last)
+    qcdev::LexLess less{off, keys};
+    tb = P.temp_bytes;
+    QC_RP(rocprim::merge_sort(temp, tb, U32(P.mlist), U32(P.sorted), (size_t)k, less, s));
+
+    // 6. output
+    hipLaunchKernelGGL(qcdev::nm_out_lens, dim3(gk), dim3(256), 0, s, off, U32(P.sorted), nmx_p, k, U64(P.lens));
+    QC_HIP(hipGetLastError());
+    tb = P.temp_bytes;
+    QC_RP(rocprim::exclusive_scan(temp, tb, U64(P.lens), U64(P.oo), (uint64_t)0, (size_t)k + 1,
+                                  rocprim::plus<uint64_t>(), s));
+    hipLaunchKernelGGL(qcdev::nm_out_write, dim3(gw), dim3(64), 0, s, off, keys, U32(P.inst), in.base,
+                       U32(P.sorted), nmx_p, U64(P.oo), err, out.count, out.off, out.v, out.inst);
+    QC_HIP(hipGetLastError());
+    return QC_OK;
+}
+
+int remove_nonmax_sets(const NmInput& in, int n_inst, std::vector<std::vector<uint64_t>*> mo,
+                       std::vector<std::vector<uint32_t>*> mv, hipStream_t s) {
+    int rc;
+    for (int i = 0; i < n_inst; i++) {
+        mo[i]->assign(1, 0);
+        mv[i]->clear();
+    }
+    if (in.k == 0) return QC_OK;
+    const size_t need = nonmax_scratch_bytes(in.k, in.nv, std::max<uint32_t>(in.nkeys, 1), s);
+    if (!need) return QC_EDEVICE;
+    DBuf scr, d_count, d_off, d_v, d_inst;
+    if ((rc = scr.alloc(need, s)) || (rc = d_count.alloc(8, s)) || (rc = d_off.alloc((in.k + 1) * 8, s)) ||
+        (rc = d_v.alloc(in.nv * 4, s)) || (rc = d_inst.alloc(in.k * 4, s)))
+        return rc;
+    NmOutput out{d_count.as<uint64_t>(), d_off.as<uint64_t>(), d_v.as<uint32_t>(), d_inst.as<uint32_t>()};
+    if ((rc = nonmax_run(in, scr.as<char>(), need, out, s))) return rc;
+    uint64_t cnt = 0;
+    QC_HIP(hipMemcpyAsync(&cnt, out.count, 8, hipMemcpyDeviceToHost, s));
+    QC_HIP(hipStreamSynchronize(s));
+    if (cnt == ~0ull) {
+        set_error("RemoveNonMax: a raw set holds a vertex outside its graph");
+        return QC_EINVAL;
+    }
+    std::vector<uint64_t> ho(cnt + 1);
+    std::vector<uint32_t> hinst(cnt);
+    QC_HIP(hipMemcpyAsync(ho.data(), out.off, (cnt + 1) * 8, hipMemcpyDeviceToHost, s));
+    QC_HIP(hipMemcpyAsync(hinst.data(), out.inst, cnt * 4, hipMemcpyDeviceToHost, s));
+    QC_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> hv(ho[cnt]);
+    QC_HIP(hipMemcpyAsync(hv.data(), out.v, ho[cnt] * 4, hipMemcpyDeviceToHost, s));
+    QC_HIP(hipStreamSynchronize(s));
+    for (uint64_t r = 0; r < cnt; r++) {
+        const uint32_t i = hinst[r];
+        if ((int)i >= n_inst) {
+            set_error("RemoveNonMax: instance %u out of range", i);
+            return QC_EDEVICE;
+        }
+        mv[i]->insert(mv[i]->end(), hv.begin() + ho[r], hv.begin() + ho[r + 1]);
+        mo[i]->push_back(mv[i]->size());
+    }
+    return QC_OK;
+}
+
 int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uint32_t>& v, std::vector<uint64_t>& mo,
                          std::vector<uint32_t>& mv, hipStream_t s) {
     int rc;
@@ -184,137 +547,25 @@ int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uin
     mv.clear();
     const uint64_t k = off.size() - 1;
     if (k == 0) return QC_OK;
-    if (k >= 0xFFFFFFFFull) {
-        set_error("too many raw sets (%llu)", (unsigned long long)k);
-        return QC_EOVERFLOW;
-    }
     const uint64_t nv = v.size();
     uint32_t maxv = 0;
     for (uint32_t x : v) maxv = std::max(maxv, x);
-    const uint32_t nvert = maxv + 1;
-
-    DBuf d_off, d_vin, d_v, d_h, d_hs, d_idx, d_ids, d_keep, d_pos, d_uniq, d_cnt, d_start, d_cur, d_post, d_max,
-        d_mpos, d_mlist, d_sorted, d_tmp;
-    if ((rc = d_off.alloc((k + 1) * 8))) return rc;
-    if ((rc = d_vin.alloc(nv * 4))) return rc;
-    if ((rc = d_v.alloc(nv * 4))) return rc;
+    if (maxv == 0xFFFFFFFFu) {
+        set_error("vertex id 4294967295 is not supported");
+        return QC_EINVAL;
+    }
+    DBuf d_off, d_v;
+    if ((rc = d_off.alloc((k + 1) * 8, s)) || (rc = d_v.alloc(nv * 4, s))) return rc;
     QC_HIP(hipMemcpyAsync(d_off.p, off.data(), (k + 1) * 8, hipMemcpyHostToDevice, s));
-    QC_HIP(hipMemcpyAsync(d_vin.p, v.data(), nv * 4, hipMemcpyHostToDevice, s));
-
-    // 1. sort every set ascending
-    size_t tb = 0;
-    QC_RP(rocprim::segmented_radix_sort_keys(nullptr, tb, d_vin.as<uint32_t>(), d_v.as<uint32_t>(), (unsigned int)nv,
-                                            (unsigned int)k, d_off.as<uint64_t>(), d_off.as<uint64_t>() + 1, 0, 32,
-                                            s));
-    if ((rc = d_tmp.alloc(tb))) return rc;
-    QC_RP(rocprim::segmented_radix_sort_keys(d_tmp.p, tb, d_vin.as<uint32_t>(), d_v.as<uint32_t>(), (unsigned int)nv,
-                                            (unsigned int)k, d_off.as<uint64_t>(), d_off.as<uint64_t>() + 1, 0, 32,
-                                            s));
-    d_vin.release();
-
-    // 2. hash, sort by hash, dedupe
-    if ((rc = d_h.alloc(k * 8)) || (rc = d_hs.alloc(k * 8)) || (rc = d_idx.alloc(k * 4)) ||
-        (rc = d_ids.alloc(k * 4)) || (rc = d_keep.alloc(k * 4)) || (rc = d_pos.alloc(k * 4)))
-        return rc;
-    const unsigned gw = (unsigned)std::min<uint64_t>(k, 65536);
-    hipLaunchKernelGGL(qcdev::nm_hash, dim3(gw), dim3(64), 0, s, d_off.as<uint64_t>(), d_v.as<uint32_t>(), k,
-                       d_h.as<uint64_t>(), d_idx.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-    tb = 0;
-    QC_RP(rocprim::radix_sort_pairs(nullptr, tb, d_h.as<uint64_t>(), d_hs.as<uint64_t>(), d_idx.as<uint32_t>(),
-                                    d_ids.as<uint32_t>(), (size_t)k, 0, 64, s));
-    if (tb > d_tmp.bytes && (rc = d_tmp.alloc(tb))) return rc;
-    QC_RP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_h.as<uint64_t>(), d_hs.as<uint64_t>(), d_idx.as<uint32_t>(),
-                                    d_ids.as<uint32_t>(), (size_t)k, 0, 64, s));
-    const unsigned gb = (unsigned)std::min<uint64_t>((k + 255) / 256, 16384);
-    hipLaunchKernelGGL(qcdev::nm_dedupe, dim3(gb), dim3(256), 0, s, d_off.as<uint64_t>(), d_v.as<uint32_t>(),
-                       d_hs.as<uint64_t>(), d_ids.as<uint32_t>(), k, d_keep.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-    tb = 0;
-    QC_RP(rocprim::exclusive_scan(nullptr, tb, d_keep.as<uint32_t>(), d_pos.as<uint32_t>(), 0u, (size_t)k,
-                                  rocprim::plus<uint32_t>(), s));
-    if (tb > d_tmp.bytes && (rc = d_tmp.alloc(tb))) return rc;
-    QC_RP(rocprim::exclusive_scan(d_tmp.p, tb, d_keep.as<uint32_t>(), d_pos.as<uint32_t>(), 0u, (size_t)k,
-                                  rocprim::plus<uint32_t>(), s));
-    uint32_t last_pos = 0, last_keep = 0;
-    QC_HIP(hipMemcpyAsync(&last_pos, d_pos.as<uint32_t>() + (k - 1), 4, hipMemcpyDeviceToHost, s));
-    QC_HIP(hipMemcpyAsync(&last_keep, d_keep.as<uint32_t>() + (k - 1), 4, hipMemcpyDeviceToHost, s));
-    QC_HIP(hipStreamSynchronize(s));
-    const uint32_t nu = last_pos + last_keep;
-    if ((rc = d_uniq.alloc((size_t)nu * 4))) return rc;
-    hipLaunchKernelGGL(qcdev::nm_gather_unique, dim3(gb), dim3(256), 0, s, d_ids.as<uint32_t>(),
-                       d_keep.as<uint32_t>(), d_pos.as<uint32_t>(), k, d_uniq.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-
-    // 3. inverted index
-    if ((rc = d_cnt.alloc((size_t)nvert * 4)) || (rc = d_start.alloc((size_t)nvert * 4)) ||
-        (rc = d_cur.alloc((size_t)nvert * 4)))
-        return rc;
-    QC_HIP(hipMemsetAsync(d_cnt.p, 0, (size_t)nvert * 4, s));
-    QC_HIP(hipMemsetAsync(d_cur.p, 0, (size_t)nvert * 4, s));
-    const unsigned gu = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>(nu, 65536));
-    hipLaunchKernelGGL(qcdev::nm_count, dim3(gu), dim3(64), 0, s, d_off.as<uint64_t>(), d_v.as<uint32_t>(),
-                       d_uniq.as<uint32_t>(), nu, d_cnt.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-    tb = 0;
-    QC_RP(rocprim::exclusive_scan(nullptr, tb, d_cnt.as<uint32_t>(), d_start.as<uint32_t>(), 0u, (size_t)nvert,
-                                  rocprim::plus<uint32_t>(), s));
-    if (tb > d_tmp.bytes && (rc = d_tmp.alloc(tb))) return rc;
-    QC_RP(rocprim::exclusive_scan(d_tmp.p, tb, d_cnt.as<uint32_t>(), d_start.as<uint32_t>(), 0u, (size_t)nvert,
-                                  rocprim::plus<uint32_t>(), s));
-    uint64_t npost = 0;
-    {
-        uint32_t a = 0, b = 0;
-        QC_HIP(hipMemcpyAsync(&a, d_start.as<uint32_t>() + (nvert - 1), 4, hipMemcpyDeviceToHost, s));
-        QC_HIP(hipMemcpyAsync(&b, d_cnt.as<uint32_t>() + (nvert - 1), 4, hipMemcpyDeviceToHost, s));
-        QC_HIP(hipStreamSynchronize(s));
-        npost = (uint64_t)a + b;
-    }
-    if ((rc = d_post.alloc(npost * 4))) return rc;
-    hipLaunchKernelGGL(qcdev::nm_scatter, dim3(gu), dim3(64), 0, s, d_off.as<uint64_t>(), d_v.as<uint32_t>(),
-                       d_uniq.as<uint32_t>(), nu, d_start.as<uint32_t>(), d_cur.as<uint32_t>(),
-                       d_post.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-
-    // 4. subset test
-    if ((rc = d_max.alloc((size_t)nu * 4)) || (rc = d_mpos.alloc((size_t)nu * 4))) return rc;
-    hipLaunchKernelGGL(qcdev::nm_subset, dim3(gu), dim3(64), 0, s, d_off.as<uint64_t>(), d_v.as<uint32_t>(),
-                       d_uniq.as<uint32_t>(), nu, d_start.as<uint32_t>(), d_cnt.as<uint32_t>(),
-                       d_post.as<uint32_t>(), d_max.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-    tb = 0;
-    QC_RP(rocprim::exclusive_scan(nullptr, tb, d_max.as<uint32_t>(), d_mpos.as<uint32_t>(), 0u, (size_t)nu,
-                                  rocprim::plus<uint32_t>(), s));
-    if (tb > d_tmp.bytes && (rc = d_tmp.alloc(tb))) return rc;
-    QC_RP(rocprim::exclusive_scan(d_tmp.p, tb, d_max.as<uint32_t>(), d_mpos.as<uint32_t>(), 0u, (size_t)nu,
-                                  rocprim::plus<uint32_t>(), s));
-    uint32_t lp = 0, lm = 0;
-    QC_HIP(hipMemcpyAsync(&lp, d_mpos.as<uint32_t>() + (nu - 1), 4, hipMemcpyDeviceToHost, s));
-    QC_HIP(hipMemcpyAsync(&lm, d_max.as<uint32_t>() + (nu - 1), 4, hipMemcpyDeviceToHost, s));
-    QC_HIP(hipStreamSynchronize(s));
-    const uint32_t nmx = lp + lm;
-    if ((rc = d_mlist.alloc((size_t)nmx * 4)) || (rc = d_sorted.alloc((size_t)nmx * 4))) return rc;
-    hipLaunchKernelGGL(qcdev::nm_gather_unique, dim3(gb), dim3(256), 0, s, d_uniq.as<uint32_t>(),
-                       d_max.as<uint32_t>(), d_mpos.as<uint32_t>(), (uint64_t)nu, d_mlist.as<uint32_t>());
-    QC_HIP(hipGetLastError());
-
-    // 5. lexicographic order
-    qcdev::LexLess less{d_off.as<uint64_t>(), d_v.as<uint32_t>()};
-    tb = 0;
-    QC_RP(rocprim::merge_sort(nullptr, tb, d_mlist.as<uint32_t>(), d_sorted.as<uint32_t>(), (size_t)nmx, less, s));
-    if (tb > d_tmp.bytes && (rc = d_tmp.alloc(tb))) return rc;
-    QC_RP(rocprim::merge_sort(d_tmp.p, tb, d_mlist.as<uint32_t>(), d_sorted.as<uint32_t>(), (size_t)nmx, less, s));
-
-    std::vector<uint32_t> order(nmx), sv(nv);
-    QC_HIP(hipMemcpyAsync(order.data(), d_sorted.p, (size_t)nmx * 4, hipMemcpyDeviceToHost, s));
-    QC_HIP(hipMemcpyAsync(sv.data(), d_v.p, nv * 4, hipMemcpyDeviceToHost, s));
-    QC_HIP(hipStreamSynchronize(s));
-    mo.reserve(nmx + 1);
-    for (uint32_t x : order) {
-        mv.insert(mv.end(), sv.begin() + off[x], sv.begin() + off[x + 1]);
-        mo.push_back(mv.size());
-    }
-    return QC_OK;
+    QC_HIP(hipMemcpyAsync(d_v.p, v.data(), nv * 4, hipMemcpyHostToDevice, s));
+    NmInput in{};
+    in.k = k;
+    in.nv = nv;
+    in.offsets = d_off.as<uint64_t>();
+    in.verts = d_v.as<uint32_t>();
+    in.nkeys = maxv + 1;
+    return remove_nonmax_sets(in, 1, {&mo}, {&mv}, s);
 }
 
 }  // namespace qc
+

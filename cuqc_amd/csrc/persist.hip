@@ -11,6 +11,7 @@
 // replayed.
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -22,19 +23,24 @@ namespace qcdev {
 template <int TL>
 __global__ void expand_persistent(const ExpandArgs*);
 
-// Queue state for a fresh run: roots in slots [0, nroots), their records at
-// the start of the pool, every other ready flag cleared, counters reset.
-// Roots go to the top bucket nb - 1; ctr = heads[nb], tails[nb], pool top,
-// pending, QSTRIDE words apart.
+// Queue state for a fresh run: the roots at positions [0, nroots) of the
+// top ring (published), every other cell free for its first lap, the free
+// lists empty, counters reset.  ctr = heads[nb], tails[nb], pool top,
+// pending, overflow, QSTRIDE words apart.  Root records sit at the start of
+// the pool; allocated blocks start at the next 64-Vrec boundary.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
-                              uint64_t root_slots, TaskRef* q_refs, unsigned int* q_ready, uint64_t q_cap, int nb,
-                              Vrec* pool, unsigned long long* ctr, StreamCtr* clq_ctr, DevStats* stats,
-                              DevStats* inst_stats, uint32_t n_inst, int* error, const float* __restrict__ root_feat,
-                              float* trace_feat, long long* trace_parent, unsigned int* trace_beyond) {
+                              uint64_t root_slots, TaskRef* q_refs, unsigned long long* q_seq, uint64_t q_cap, int nb,
+                              Vrec* pool, unsigned long long* ctr, unsigned long long* fl_head, StreamCtr* clq_ctr,
+                              DevStats* stats, DevStats* inst_stats, uint32_t n_inst, int* error,
+                              const float* __restrict__ root_feat, float* trace_feat, long long* trace_parent,
+                              unsigned int* trace_beyond) {
     const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t top = (uint64_t)(nb - 1) * q_cap;
-    for (uint64_t i = tid; i < (uint64_t)nb * q_cap; i += nt) q_ready[i] = (i >= top && i < top + nroots) ? 1u : 0u;
+    for (uint64_t i = tid; i < (uint64_t)nb * q_cap; i += nt) {
+        const uint64_t pos = i % q_cap;
+        q_seq[i] = (i >= top && pos < nroots) ? pos + 1 : pos;
+    }
     for (uint64_t i = tid; i < nroots; i += nt) q_refs[top + i] = roots[i];
     if (trace_feat) {
         for (uint64_t i = tid; i < (uint64_t)nroots * 12; i += nt) trace_feat[i] = root_feat[i];
@@ -42,14 +48,16 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
         for (uint64_t i = tid; i < q_cap; i += nt) trace_beyond[i] = 0;
     }
     for (uint64_t i = tid; i < root_slots; i += nt) pool[i] = root_verts[i];
+    for (uint64_t i = tid; i < (uint64_t)POOL_NCLASS; i += nt) fl_head[i * QSTRIDE] = (1ull << 36) - 1;
     for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (tid == 0) {
         for (int b = 0; b < nb; b++) {
             ctr[b * QSTRIDE] = 0;                                  // head
             ctr[(nb + b) * QSTRIDE] = b == nb - 1 ? nroots : 0;    // tail
         }
-        ctr[2 * nb * QSTRIDE] = root_slots;      // pool top
-        ctr[(2 * nb + 1) * QSTRIDE] = nroots;    // pending
+        ctr[2 * nb * QSTRIDE] = (root_slots + 63) / 64 * 64;  // pool top
+        ctr[(2 * nb + 1) * QSTRIDE] = nroots;                 // pending
+        ctr[(2 * nb + 2) * QSTRIDE] = 0;                      // overflow
         clq_ctr->items = 0;
         clq_ctr->slots = 0;
         *stats = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -69,8 +77,10 @@ size_t ws_bytes_host(uint32_t tcap, bool mat, uint32_t mwords) {
     return (ws_bytes(tcap, mat, mwords) + 255) & ~(size_t)255;
 }
 
+constexpr int NCTR = 11;  // heads, tails (4 buckets at most), pool top, pending, overflow
+
 struct RunState {
-    unsigned long long ctr[10];
+    unsigned long long ctr[NCTR];
     StreamCtr clq;
     int err;
 };
@@ -80,8 +90,8 @@ struct RunState {
 // serialize on the mutex.  Never destroyed: the runtime may be gone at exit.
 struct Workspace {
     std::mutex mu;
-    DBuf pool, q, ready, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args;
-    uint64_t pool_cap = 0, q_cap = 0, clq_slots = 0, clq_refs_cap = 0;
+    DBuf pool, q, seq, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args, fl, priv, pbkt;
+    uint64_t pool_cap = 0, q_cap = 0, clq_slots = 0, clq_refs_cap = 0, priv_cap = 0;
 
     static Workspace& get(int dev) {
         static std::mutex m;
@@ -92,6 +102,10 @@ struct Workspace {
         return *all[dev];
     }
 };
+
+double now_steady_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 int ensure(DBuf& b, size_t bytes) {
     if (b.p && b.bytes >= bytes) return QC_OK;
@@ -119,6 +133,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     const int nb = cost ? 4 : 1;
     if (cost && p.trace) {
         set_error("task traces are recorded with FIFO order only");
+        return QC_EINVAL;
+    }
+    const bool stat = p.scheduling != 0;
+    if (cost && stat) {
+        set_error("static scheduling deals ring positions to the waves in turn; cost order needs dynamic scheduling");
         return QC_EINVAL;
     }
     const void* kern = TL == 64    ? (const void*)qcdev::expand_persistent<64>
@@ -184,17 +203,22 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
                                    use_mid ? ws_bytes_host(mid, true, midw) : (size_t)0);
     size_t freeb = 0, totalb = 0;
     QC_HIP(hipMemGetInfo(&freeb, &totalb));
+    freeb += pool_cached_bytes(dev);
 
     Workspace& ws = Workspace::get(dev);
     std::lock_guard<std::mutex> lock(ws.mu);
-    const size_t budget = std::min<size_t>((freeb + ws.pool.bytes + ws.q.bytes + ws.ready.bytes + ws.slab.bytes +
+    const size_t budget = std::min<size_t>((freeb + ws.pool.bytes + ws.q.bytes + ws.seq.bytes + ws.slab.bytes + ws.priv.bytes +
                                             ws.clq_refs.bytes + ws.clq_verts.bytes) / 2,
                                            (size_t)128 << 30);
     if ((size_t)grid * slab_b > budget / 4) grid = (uint32_t)std::max<size_t>(1, budget / 4 / slab_b);
 
-    // capacities: the FIFO keeps every record it held (no reuse); they start
-    // from what the device workspace already holds and grow by rerunning the
-    // (deterministic) launch when a run overflows, unless the caller fixed them
+    // capacities: records are recycled through the pool's free lists, the
+    // rings wrap and every wave keeps its own children depth first, so the
+    // live set is the search frontier, not every task ever written (with a
+    // trace, positions number the trace rows: the rings then never wrap and
+    // every task goes through them).  They start from what the device
+    // workspace already holds and grow by rerunning the (deterministic)
+    // launch when a run overflows, unless the caller fixed them
     const bool fixed_tasks = p.task_capacity != 0, fixed_clq = p.clique_capacity != 0;
     const uint64_t max_pool = std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec);
     const uint64_t max_q = (1ull << 28) / nb;
@@ -207,6 +231,10 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     uint64_t clq_slots = fixed_clq ? p.clique_capacity
                                    : std::max<uint64_t>(ws.clq_slots, std::min<uint64_t>(max_clq, 1ull << 24));
     uint64_t clq_refs = std::max<uint64_t>(ws.clq_refs_cap, clq_slots / 8 + 1024);
+    // private stacks: a power of two of refs per wave (the stack is a ring
+    // indexed by 32-bit counters)
+    const uint64_t max_priv = 1ull << 20;
+    uint64_t priv_cap = std::max<uint64_t>(ws.priv_cap, 8192);
     if (const char* ic = getenv("QCMINE_INITIAL_CAPACITY")) {
         // start small (tests of the grow-and-rerun path)
         const uint64_t c0 = std::max<uint64_t>(1, strtoull(ic, nullptr, 10));
@@ -214,14 +242,18 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         q_cap = std::max<uint64_t>(c0, roots.size() + 1024);
         if (!fixed_clq) clq_slots = c0;
         clq_refs = c0 / 8 + 1024;
+        priv_cap = 4;
+        while (priv_cap * 2 <= std::min<uint64_t>(c0, 8192)) priv_cap *= 2;
     }
+    // the roots' records precede the first pool block
+    pool_cap = std::max<uint64_t>(pool_cap, (root_verts.size() + 63) / 64 * 64 + 64);
 
     const bool trace = p.trace != 0;
     DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_tbey, d_rfeat, d_cmw, d_cmn, d_phase, d_tl;
     // diagnostics: QCMINE_TIMELINE=<file> writes one CSV line per task taken
     const char* timeline_path = getenv("QCMINE_TIMELINE");
     if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
-    if ((rc = ensure(ws.ctr, 10 * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
+    if ((rc = ensure(ws.ctr, NCTR * QSTRIDE * 8)) || (rc = ensure(ws.fl, (size_t)POOL_NCLASS * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
         (rc = ensure(ws.stats, sizeof(DevStats))) || (rc = ensure(ws.err, sizeof(int))) ||
         (rc = ensure(ws.args, sizeof(ExpandArgs))) || (rc = ensure(ws.slab, (size_t)grid * slab_b)) ||
         (rc = d_istats.alloc((size_t)count * sizeof(DevStats))) ||
@@ -260,8 +292,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     float first_ms = 0;
     for (int attempt = 0;; attempt++) {
         if ((rc = ensure(ws.pool, pool_cap * sizeof(Vrec))) || (rc = ensure(ws.q, nb * q_cap * sizeof(TaskRef))) ||
-            (rc = ensure(ws.ready, nb * q_cap * 4)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
-            (rc = ensure(ws.clq_verts, clq_slots * 4)))
+            (rc = ensure(ws.seq, nb * q_cap * 8)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
+            (rc = ensure(ws.clq_verts, clq_slots * 4)) || (rc = ensure(ws.priv, (size_t)grid * priv_cap * sizeof(TaskRef))) ||
+            (rc = ensure(ws.pbkt, (size_t)grid * priv_cap)))
             return rc;
         if (trace && ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8)) ||
                       (rc = d_tbey.alloc(q_cap * 4))))
@@ -270,6 +303,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         ws.q_cap = std::max<uint64_t>(ws.q_cap, nb * q_cap);
         ws.clq_slots = std::max(ws.clq_slots, clq_slots);
         ws.clq_refs_cap = std::max(ws.clq_refs_cap, clq_refs);
+        ws.priv_cap = std::max(ws.priv_cap, priv_cap);
 
         ExpandArgs A;
         memset(&A, 0, sizeof(A));
@@ -296,12 +330,19 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 64 : p.split_chunk);
         A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : 512);
         A.q_refs = ws.q.as<TaskRef>();
-        A.q_ready = ws.ready.as<unsigned int>();
+        A.q_seq = ws.seq.as<unsigned long long>();
+        A.fl_head = ws.fl.as<unsigned long long>();
+        A.priv = ws.priv.as<TaskRef>();
+        A.priv_bkt = ws.pbkt.as<unsigned char>();
+        A.priv_cap = (uint32_t)priv_cap;
+        A.use_priv = trace ? 0 : 1;
+        A.static_sched = stat ? 1 : 0;
         unsigned long long* ctr = ws.ctr.as<unsigned long long>();
         A.q_head = ctr;
         A.q_tail = ctr + nb * QSTRIDE;
         A.pool_top = ctr + 2 * nb * QSTRIDE;
         A.pending = (long long*)(ctr + (2 * nb + 1) * QSTRIDE);
+        A.overflow = ctr + (2 * nb + 2) * QSTRIDE;
         A.n_buckets = nb;
         A.n_roots = (int32_t)roots.size();
         A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
@@ -330,8 +371,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
                            d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), ws.q.as<TaskRef>(),
-                           ws.ready.as<unsigned int>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
-                           ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
+                           ws.seq.as<unsigned long long>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
+                           ws.fl.as<unsigned long long>(), ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
                            (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent,
                            A.trace_beyond);
         hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, lds_b, s);
@@ -368,9 +409,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
             RunState rs{};
             std::vector<DevStats> ist(count);
             {
-                unsigned long long raw[10 * QSTRIDE];
+                unsigned long long raw[NCTR * QSTRIDE];
                 QC_HIP(hipMemcpy(raw, ctr, sizeof(raw), hipMemcpyDeviceToHost));
-                for (int k = 0; k < 10; k++) rs.ctr[k] = raw[k * QSTRIDE];
+                for (int k = 0; k < NCTR; k++) rs.ctr[k] = raw[k * QSTRIDE];
             }
             QC_HIP(hipMemcpy(&rs.clq, ws.clq_ctr.p, sizeof(rs.clq), hipMemcpyDeviceToHost));
             QC_HIP(hipMemcpy(&rs.err, ws.err.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -379,16 +420,23 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
                 if (rs.err == 1 && rep == 0) {
                     // grow what ran out (its counter kept counting) and rerun
                     const uint64_t pool_need = rs.ctr[2 * nb];
-                    uint64_t q_need = 0;
-                    for (int b = 0; b < nb; b++) q_need = std::max<uint64_t>(q_need, rs.ctr[nb + b]);
+                    const uint64_t pushes_lost = rs.ctr[2 * nb + 2];
                     bool grown = false;
                     if (pool_need > pool_cap && !fixed_tasks && pool_cap < max_pool) {
                         pool_cap = std::min<uint64_t>(max_pool, std::max<uint64_t>(pool_cap * 4, pool_need * 2));
                         grown = true;
                     }
-                    if (q_need > q_cap && q_cap < max_q) {
-                        q_cap = std::min<uint64_t>(max_q, std::max<uint64_t>(q_cap * 4, q_need * 2));
-                        grown = true;
+                    // a push found its ring and this wave's stack full (with
+                    // a trace: the ring, which does not wrap)
+                    if (pushes_lost > 0) {
+                        if (q_cap < max_q) {
+                            q_cap = std::min<uint64_t>(max_q, q_cap * 4);
+                            grown = true;
+                        }
+                        if (!trace && priv_cap < max_priv) {
+                            priv_cap = std::min<uint64_t>(max_priv, priv_cap * 4);
+                            grown = true;
+                        }
                     }
                     if ((rs.clq.slots > clq_slots || rs.clq.items > clq_refs) && !fixed_clq && clq_slots < max_clq) {
                         clq_slots = std::min<uint64_t>(max_clq, std::max<uint64_t>(clq_slots * 4, rs.clq.slots * 2));
@@ -410,8 +458,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
                 first = rs;
                 first_stats = ist;
                 first_ms = ms;
-            } else if (memcmp(first.ctr, rs.ctr, sizeof(rs.ctr)) != 0 || first.clq.items != rs.clq.items ||
-                       first.clq.slots != rs.clq.slots ||
+            } else if (first.clq.items != rs.clq.items || first.clq.slots != rs.clq.slots ||
                        memcmp(first_stats.data(), ist.data(), (size_t)count * sizeof(DevStats)) != 0) {
                 set_error("graph replay %d differs from the first launch", rep);
                 status = QC_EDEVICE;
@@ -474,6 +521,49 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         outs[i].ds = first_stats[i];
         outs[i].kernel_ms = first_ms;
         outs[i].levels = 1;
+        outs[i].pool_peak = first.ctr[2 * nb];
+    }
+    // the ENUMERATION window ends with the raw sets collected; RemoveNonMax
+    // then runs on the raw sets where the kernel left them, all instances in
+    // one pass (instance i's vertices are keyed from base[i])
+    const double t_enum_end = now_steady_ms();
+    for (int i = 0; i < count; i++) outs[i].t_enum_end_ms = t_enum_end;
+    if (cc.items) {
+        std::vector<uint32_t> base(count), limit(count);
+        uint64_t nk = 0;
+        for (int i = 0; i < count; i++) {
+            base[i] = (uint32_t)nk;
+            limit[i] = (uint32_t)insts[i]->n;
+            nk += (uint64_t)insts[i]->n;
+        }
+        if (nk >= 0xFFFFFFFFull) {
+            set_error("RemoveNonMax: the batch holds more than 2^32 - 1 vertices");
+            return QC_EOVERFLOW;
+        }
+        DBuf d_base, d_limit;
+        if ((rc = d_base.alloc((size_t)count * 4, s)) || (rc = d_limit.alloc((size_t)count * 4, s))) return rc;
+        QC_HIP(hipMemcpyAsync(d_base.p, base.data(), (size_t)count * 4, hipMemcpyHostToDevice, s));
+        QC_HIP(hipMemcpyAsync(d_limit.p, limit.data(), (size_t)count * 4, hipMemcpyHostToDevice, s));
+        NmInput in;
+        in.k = cc.items;
+        in.nv = cc.slots;
+        in.refs = ws.clq_refs.as<CliqueRef>();
+        in.verts = ws.clq_verts.as<uint32_t>();
+        in.base = d_base.as<uint32_t>();
+        in.limit = d_limit.as<uint32_t>();
+        in.nkeys = (uint32_t)std::max<uint64_t>(nk, 1);
+        std::vector<std::vector<uint64_t>*> mo(count);
+        std::vector<std::vector<uint32_t>*> mv(count);
+        for (int i = 0; i < count; i++) {
+            mo[i] = &outs[i].max_off;
+            mv[i] = &outs[i].max_v;
+        }
+        if ((rc = remove_nonmax_sets(in, count, mo, mv, s))) return rc;
+    }
+    const double nm_ms = now_steady_ms() - t_enum_end;
+    for (int i = 0; i < count; i++) {
+        outs[i].have_max = true;
+        outs[i].nonmax_ms = nm_ms;
     }
     if (trace) {
         // rows = expanded tasks (split items excluded), per instance; a

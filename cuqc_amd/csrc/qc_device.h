@@ -14,6 +14,26 @@
 // stride (in 64-bit words) between the persistent queue's shared counters
 constexpr int QSTRIDE = 32;
 
+// Vertex pool of the persistent form: records live in blocks of whole
+// 64-Vrec units, in size classes of 1..8 units and then four steps per
+// octave (10, 12, 14, 16, 20, 24, ...); a freed block goes on its class's
+// lock-free free list and is reused by the next record of that class.
+constexpr int POOL_NCLASS = 104;
+__host__ __device__ inline int pool_class(uint32_t nvrec, uint32_t& units) {
+    uint32_t u = (nvrec + 63u) / 64u;
+    if (u == 0) u = 1;
+    if (u <= 8) {
+        units = u;
+        return (int)u - 1;
+    }
+    int e = 0;  // u - 1 in [2^e, 2^(e+1)), e >= 3
+    for (uint32_t x = u - 1; x > 1; x >>= 1) e++;
+    const uint32_t step = 1u << (e - 2);
+    const uint32_t k = (u - (1u << e) + step - 1) / step;  // 1..4
+    units = (1u << e) + k * step;
+    return 8 + (e - 3) * 4 + (int)(k - 1);
+}
+
 __host__ __device__ constexpr uint32_t pow2_at_least(uint32_t x) {
     uint32_t p = 1;
     while (p < x) p <<= 1;
@@ -140,26 +160,40 @@ struct ExpandArgs {
     int32_t lds_masks;               // persistent mode: byte offset in the dynamic LDS of the
                                      // two set masks of tasks above the LDS tile (-1: slab)
 
-    // persistent mode: n_buckets FIFO queues of task refs (bucket b owns
-    // slots [b q_cap, (b+1) q_cap) of q_refs / q_ready and the counters
-    // q_head[b QSTRIDE], q_tail[b QSTRIDE]; every shared counter sits on a
-    // 256-byte line of its own) over a bump-allocated vertex pool; q_ready[i]
-    // publishes slot i; `pending` counts tasks pushed and not yet finished
-    // (0 = search complete).  With a cost model, a child goes to the bucket
-    // of its predicted log subtree size and waves pop the highest non-empty
+    // persistent mode: n_buckets shared ring queues of task refs plus one
+    // private LIFO stack per wave, over a vertex pool with free lists.
+    // Bucket b owns cells [b q_cap, (b+1) q_cap) of q_refs / q_seq and the
+    // position counters q_head[b QSTRIDE], q_tail[b QSTRIDE] (every shared
+    // counter sits on a 256-byte line of its own).  Position p of a bucket
+    // uses cell p mod q_cap, whose sequence word reads p while the cell is
+    // free for p, p + 1 once the ref of p is published and p + q_cap once it
+    // has been taken.  `pending` counts tasks pushed anywhere and not yet
+    // finished (0 = search complete).  With a cost model, a child's bucket
+    // is its predicted log subtree size; waves take the highest non-empty
     // bucket first.
     int32_t queue_mode;
     int32_t split_chunk;             // branches per split item (0: no splitting)
     uint32_t split_min_t;            // split tasks above this size
     TaskRef* q_refs;
-    unsigned int* q_ready;
+    unsigned long long* q_seq;
     unsigned long long* q_head;
     unsigned long long* q_tail;
     unsigned long long q_cap;
     Vrec* pool;
-    unsigned long long* pool_top;
+    unsigned long long* pool_top;    // bump pointer for blocks no free list holds (Vrec units)
     unsigned long long pool_cap;
+    unsigned long long* fl_head;     // free-list heads, POOL_NCLASS lines of QSTRIDE words:
+                                     // (tag << 36) | block index (64-Vrec units), all ones = empty
     long long* pending;
+    unsigned long long* overflow;    // pushes that found the ring and the private stack full
+    // private stacks: wave w owns priv[w priv_cap, (w+1) priv_cap) (refs)
+    // and priv_bkt[...] (the bucket each ref would take in the rings);
+    // children stay there (depth first) unless use_priv is 0, and the
+    // oldest ones are handed to the rings while they run low
+    TaskRef* priv;
+    unsigned char* priv_bkt;
+    uint32_t priv_cap;
+    int32_t use_priv;
 
     // task trace (persistent mode, NULL = off): features of the task in
     // queue slot i at trace_feat[12 i], its parent's slot at trace_parent[i]

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "qc_device.h"
@@ -39,8 +40,10 @@ const char* last_error();
 // Device blocks up to POOL_MAX bytes are recycled through a per-device
 // cache (power-of-two classes) instead of a hipMalloc / hipFree pair per
 // call; larger ones are allocated and freed directly.  A block goes back to
-// the cache only when its owner releases it, after the work using it has
-// been synchronized.
+// the cache only once no enqueued work uses it: an owner that names the
+// stream its work runs on has that stream synchronized on release; a block
+// allocated without a stream is released only after its owner synchronized
+// the work using it.
 constexpr size_t POOL_MAX = (size_t)64 << 20;
 inline size_t pool_class(size_t b) {
     if (b > POOL_MAX) return 0;
@@ -49,7 +52,11 @@ inline size_t pool_class(size_t b) {
     return c;
 }
 void* pool_get(size_t cls, int dev, hipError_t* err);
-void pool_put(void* p, size_t cls, int dev);
+void pool_put(void* p, size_t cls, int dev, hipStream_t s);
+// frees every cached block of `dev` (after a failed allocation)
+void pool_trim(int dev);
+// bytes held by the cache for `dev` (free to reuse; hipMemGetInfo misses them)
+size_t pool_cached_bytes(int dev);
 
 // owning device buffer
 struct DBuf {
@@ -57,6 +64,7 @@ struct DBuf {
     size_t bytes = 0;
     size_t cls = 0;  // pool class, 0 = not pooled
     int dev = 0;
+    hipStream_t stream = nullptr;  // stream the buffer's work runs on (nullptr: synchronized by the owner)
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
@@ -64,24 +72,36 @@ struct DBuf {
     void release() {
         if (p) {
             if (cls)
-                pool_put(p, cls, dev);
+                pool_put(p, cls, dev, stream);
             else
                 (void)hipFree(p);
         }
         p = nullptr;
         bytes = 0;
         cls = 0;
+        stream = nullptr;
     }
-    int alloc(size_t b) {
+    // `s`: the stream that reads and writes the buffer; its release then
+    // waits for that stream, so an early release (an error return while
+    // kernels still run) is safe.  The stream must outlive the buffer.
+    int alloc(size_t b, hipStream_t s = nullptr) {
         release();
         if (b == 0) b = 16;
         (void)hipGetDevice(&dev);
         const size_t c = pool_class(b);
         hipError_t e = hipSuccess;
-        if (c)
+        if (c) {
             p = pool_get(c, dev, &e);
-        else
+        } else {
             e = hipMalloc(&p, b);
+            if (e != hipSuccess) {
+                // cached blocks are not counted as free by the runtime
+                (void)hipGetLastError();
+                pool_trim(dev);
+                p = nullptr;
+                e = hipMalloc(&p, b);
+            }
+        }
         if (e != hipSuccess || !p) {
             p = nullptr;
             set_error("hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
@@ -89,7 +109,16 @@ struct DBuf {
         }
         bytes = b;
         cls = c;
+        stream = s;
         return QC_OK;
+    }
+    // exchanges the two buffers (pointer, size, pool class, device, stream)
+    void swap(DBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        std::swap(cls, o.cls);
+        std::swap(dev, o.dev);
+        std::swap(stream, o.stream);
     }
     template <class T>
     T* as() const {
@@ -129,13 +158,53 @@ struct EnumOut {
     DevStats ds{};
     double kernel_ms = 0;
     uint64_t levels = 0;
+    uint64_t pool_peak = 0;  // persistent form: pool high-water mark (vertex slots)
+    // maximal sets, when the enumeration already removed the non-maximal
+    // ones from its device buffers (persistent form)
+    bool have_max = false;
+    std::vector<uint64_t> max_off{0};
+    std::vector<uint32_t> max_v;
+    double t_enum_end_ms = 0;  // steady-clock ms at the end of the ENUMERATION window (0: not set)
+    double nonmax_ms = 0;
 };
 int enumerate_plain(const Instance& inst, const qc_params& p, EnumOut& out, hipStream_t s);
 
 // cost model weights / norms in device memory (mlp.hip)
 int model_device_params(const qc_model* m, DBuf& pb, DBuf& norm, hipStream_t s);
 
-// RemoveNonMax on the device (nonmax.hip)
+// RemoveNonMax on the device (nonmax.hip).  Raw sets in device memory:
+// `refs` (k CliqueRefs into `verts`) or, with refs == nullptr, the sets
+// [offsets[i], offsets[i+1]) of `verts`.  With `base` set, a vertex v of
+// instance i must be below limit[i] and is compared as the key base[i] + v
+// (base, limit: device arrays); without it, vertices are below nkeys.
+// nkeys bounds every key.
+struct NmInput {
+    uint64_t k = 0;   // sets
+    uint64_t nv = 0;  // vertices over all sets
+    const CliqueRef* refs = nullptr;
+    const uint64_t* offsets = nullptr;
+    const uint32_t* verts = nullptr;
+    const uint32_t* base = nullptr;
+    const uint32_t* limit = nullptr;
+    uint32_t nkeys = 0;
+};
+// Device outputs: the count (~0 when a vertex was out of range), offsets
+// (count + 1), vertices, and the instance of every set (may be nullptr).
+// Capacities: k + 1 offsets, nv vertices, k instances.
+struct NmOutput {
+    uint64_t* count;
+    uint64_t* off;
+    uint32_t* v;
+    uint32_t* inst;
+};
+// scratch bytes nonmax_run needs for k sets, nv vertices and nkeys keys
+size_t nonmax_scratch_bytes(uint64_t k, uint64_t nv, uint32_t nkeys, hipStream_t s);
+// enqueues the whole RemoveNonMax on `s`: no allocation, no synchronization
+int nonmax_run(const NmInput& in, char* scratch, size_t scratch_bytes, const NmOutput& out, hipStream_t s);
+// nonmax_run in allocated scratch, results copied to the host per instance
+int remove_nonmax_sets(const NmInput& in, int n_inst, std::vector<std::vector<uint64_t>*> mo,
+                       std::vector<std::vector<uint32_t>*> mv, hipStream_t s);
+// host raw sets (uploaded once)
 int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uint32_t>& v, std::vector<uint64_t>& mo,
                          std::vector<uint32_t>& mv, hipStream_t s);
 

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -267,8 +268,12 @@ struct DevCSR {
 };
 void free_devcsr(void* p) { delete (DevCSR*)p; }
 
-// the graph's arrays in HBM, uploaded once per graph
+// the graph's arrays in HBM, uploaded once per graph (the batch builders
+// run on several host threads and a batch may hold one graph twice: the
+// check and the upload happen under one lock)
 int device_csr(const qc_graph* g, DevCSR** out, hipStream_t s) {
+    static std::mutex upload_mu;
+    std::lock_guard<std::mutex> lock(upload_mu);
     qc_graph* mg = const_cast<qc_graph*>(g);
     if (!mg->dev) {
         DevCSR* d = new DevCSR;
@@ -315,7 +320,8 @@ int upload_graph(const qc_graph* g, hipStream_t s) {
     return device_csr(g, &d, s);
 }
 
-int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
+namespace {
+int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
     const auto t_start = std::chrono::steady_clock::now();
     const int n = g->n;
     I.n = n;
@@ -421,9 +427,11 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     }
     DevCSR* G = nullptr;
     if ((rc = device_csr(g, &G, s))) return rc;
+    // temporaries carry the stream: released on an error return while the
+    // kernels below still run, they are not handed out again before those end
     DBuf d_loc, d_n2, d_vis;
-    if ((rc = d_loc.alloc((size_t)n * 4))) return rc;
-    if ((rc = d_n2.alloc((size_t)R * 4))) return rc;
+    if ((rc = d_loc.alloc((size_t)n * 4, s))) return rc;
+    if ((rc = d_n2.alloc((size_t)R * 4, s))) return rc;
     if ((rc = I.d_orig.alloc((size_t)R * 4))) return rc;
     QC_HIP(hipMemcpyAsync(d_loc.p, loc.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
     QC_HIP(hipMemcpyAsync(I.d_orig.p, I.orig.data(), (size_t)R * 4, hipMemcpyHostToDevice, s));
@@ -452,7 +460,7 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
         QC_HIP(hipLaunchKernel(rows_kernel, dim3(lblocks), dim3(qcdev::ROWS_THREADS), rargs, lds_bytes, s));
     } else {
         if (!g->has_n2) {
-            if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8))) return rc;
+            if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8, s))) return rc;
             QC_HIP(hipMemsetAsync(d_vis.p, 0, (size_t)blocks * vis_words * 8, s));
         }
         hipLaunchKernelGGL(qcdev::root_rows, dim3(blocks), dim3(64), 0, s, G->off.as<uint64_t>(),
@@ -541,6 +549,16 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
         if (x.label != 1) I.root_bound += std::min<uint64_t>((uint64_t)R, (uint64_t)I.d2r[x.id] + 1);
     }
     return QC_OK;
+}
+
+}  // namespace
+
+int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
+    const int rc = build_instance_impl(g, gamma, ms, I, s);
+    // an error can leave the root kernels running on the instance's tables:
+    // they finish before the caller may release the instance
+    if (rc != QC_OK) (void)hipStreamSynchronize(s);
+    return rc;
 }
 
 }  // namespace qc

@@ -95,7 +95,10 @@ typedef struct qc_stats {
     uint64_t beyond;         /* child records past the depth window (not expanded) */
     uint64_t root_branches;  /* branches of the root task (window range) */
     uint64_t w_adj;          /* adjacency entries the reference's list scans visit (root-restricted) */
-    uint64_t reserved[2];
+    uint64_t pool_peak;      /* persistent mode: high-water mark of the task pool (vertex slots,
+                                root records included); records are recycled, so this is the
+                                search frontier's size, not the sum of all records written */
+    uint64_t reserved;
     double t_init_ms;        /* root construction + device tables */
     double t_enum_ms;        /* ENUMERATION window (main.cu:551-685) */
     double t_nonmax_ms;      /* RemoveNonMax */

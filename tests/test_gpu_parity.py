@@ -208,3 +208,44 @@ def test_small_lds_wide(name, build, gamma, ms):
     _say(f"[gpu] small lds_wide {name}")
     res = qc.mine(g, gamma, ms, 0, mode="persistent", lds_tile=64, lds_wide=256, split_chunk=2, split_min=1)
     _compare(res, ref, name)
+
+
+def test_concurrent_calls_on_two_threads(cfg0):
+    """Two host threads mine at once (the library's per-device block cache and
+    workspace are shared between them): each result equals the oracle's.  One
+    thread runs batches that hold the same graph twice (its device copy is
+    made once, under a lock)."""
+    import threading
+    g, ref = cfg0
+    cases = [(n, *b(), gm, ms) for n, b, gm, ms in SMALL_CASES[:4]]
+    refs = [oracle.mine(off, adj, gm, ms) for _, off, adj, gm, ms in cases]
+    errors = []
+
+    def plain_worker():
+        try:
+            for it in range(3):
+                for (name, off, adj, gm, ms), r in zip(cases, refs):
+                    res = qc.mine(qc.Graph.from_csr(off, adj), gm, ms, 0, mode="plain")
+                    _compare(res, r, f"thread plain {name} #{it}")
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    def batch_worker():
+        try:
+            for it in range(3):
+                gg = qc.Graph.generate(0, 1)
+                out = qc.mine_batch([gg, gg], [gg.gamma] * 2, [gg.min_size] * 2, split_chunk=4, split_min=1)
+                for r in out:
+                    _compare(r, ref, f"thread batch #{it}")
+        except Exception as e:
+            errors.append(e)
+
+    _say("[gpu] two threads: plain calls and batches at once")
+    ts = [threading.Thread(target=plain_worker), threading.Thread(target=batch_worker)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=100)
+    assert not any(t.is_alive() for t in ts), "a mining thread did not finish"
+    if errors:
+        raise errors[0]
