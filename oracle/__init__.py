@@ -18,7 +18,10 @@ _lib = None
 
 STAT_FIELDS = ["tasks", "branches", "children", "raw_cliques", "sum_t", "sum_t_written", "sum_clique_len", "w_adj",
                "lookahead_hits", "stale_oob", "stale_hits", "max_t", "max_depth", "root_size", "init_w_adj", "beyond",
-               "root_branches"]
+               "root_branches",
+               # how often each pruning rule fired
+               "remove_breaks", "lu_low", "lu_lb_invalid", "lu_sum_invalid", "lu_ub_tightened", "lu_invalid",
+               "member_fail", "dp_recompute", "dp_decrement", "crit_none", "crit_fail", "crit_ok"]
 
 
 class _Stats(C.Structure):
@@ -59,6 +62,8 @@ def lib() -> C.CDLL:
         L.orc_nonmax.argtypes = [C.c_uint64, P, P]
         L.orc_build_n2.restype = C.c_uint64
         L.orc_build_n2.argtypes = [C.c_int, P, P, P, P]
+        L.orc_root.restype = C.c_int
+        L.orc_root.argtypes = [C.c_int, P, P, C.c_double, C.c_int, P, P, P]
         L.orc_brute.restype = P
         L.orc_brute.argtypes = [C.c_int, P, P, C.c_double, C.c_int]
         _lib = L
@@ -153,6 +158,18 @@ def build_n2(off, adj) -> tuple[np.ndarray, np.ndarray]:
     adj2 = np.zeros(max(int(tot), 1), np.int32)
     L.orc_build_n2(n, off.ctypes.data, adj.ctypes.data if adj.size else None, off2.ctypes.data, adj2.ctypes.data)
     return off2, adj2[:tot]
+
+
+def root(off, adj, gamma: float, min_size: int) -> dict:
+    """The root task record (initialize_tasks, main.cu:846-998) in its Q
+    order: vertex ids, labels (0 candidate, 2 covered, 3 cover), exdegs."""
+    L = lib()
+    off, adj = _csr(off, adj)
+    n = len(off) - 1
+    ids, labels, ex = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+    k = L.orc_root(n, off.ctypes.data, adj.ctypes.data if adj.size else None, float(gamma), int(min_size),
+                   ids.ctypes.data, labels.ctypes.data, ex.ctypes.data)
+    return {"ids": ids[:k].copy(), "labels": labels[:k].copy(), "exdeg": ex[:k].copy()}
 
 
 def canon(sets) -> list[tuple[int, ...]]:

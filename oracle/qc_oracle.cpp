@@ -77,6 +77,19 @@ struct Stats {
     uint64_t init_w_adj;     // entries scanned while building the root
     uint64_t beyond;         // records past the depth window (not expanded)
     uint64_t root_branches;  // branches of the root task (window range)
+    // how often each pruning rule fired (coverage of the rules by a run)
+    uint64_t remove_breaks;    // remove_one_vertex stopped at a failing member (main.cu:1437-1450)
+    uint64_t lu_low;           // LU bounds with min member indegree below mindeg[m] (main.cu:1880)
+    uint64_t lu_lb_invalid;    // no feasible lower bound (main.cu:1895-1900)
+    uint64_t lu_sum_invalid;   // the sorted-indegree sum test failed (main.cu:1932-1936)
+    uint64_t lu_ub_tightened;  // ub lowered by the sorted-indegree sums (main.cu:1946-1958)
+    uint64_t lu_invalid;       // bounds reported invalid
+    uint64_t member_fail;      // a member failed vert_isextendable_LU
+    uint64_t dp_recompute;     // degree-pruning rounds recomputing exdeg from the remaining set
+    uint64_t dp_decrement;     // rounds decrementing exdeg through the removed set
+    uint64_t crit_none;        // critical step with no critical-adjacent candidate (empty map, main.cu:1575-1602)
+    uint64_t crit_fail;        // critical candidates rejected by the adjacency checks
+    uint64_t crit_ok;          // critical candidates added as members
 };
 
 struct TraceRow {
@@ -194,7 +207,7 @@ struct Oracle {
     }
 
     // ---- LU bounds, main.cu:1857-1972 ----
-    bool lu_bounds(int& ub, int& lb, int& med, const Vtx* vs, int m, int nc, const int* ci) const {
+    bool lu_bounds(int& ub, int& lb, int& med, const Vtx* vs, int m, int nc, const int* ci) {
         bool invalid = false;
         int min_in = vs[0].indeg, min_in_ex = vs[0].exdeg;
         int min_tot = vs[0].indeg + vs[0].exdeg;
@@ -212,11 +225,13 @@ struct Oracle {
         }
         med = get_mindeg((long)m + 1);
         if (min_in < md(m)) {
+            st.lu_low++;
             lb = get_mindeg(m) - min_in;
             while (lb <= min_in_ex && min_in + lb < md((long)m + lb)) lb++;
             if (min_in + lb < md((long)m + lb)) {
                 lb = nc + 1;
                 invalid = true;
+                st.lu_lb_invalid++;
             }
             ub = (int)std::floor((double)min_tot / gamma) + 1 - m;
             if (ub > nc) ub = nc;
@@ -231,6 +246,7 @@ struct Oracle {
                 if (sum_in + sc < (long)m * md((long)m + idx)) {
                     lb = ub + 1;
                     invalid = true;
+                    st.lu_sum_invalid++;
                 } else {
                     lb = idx;
                     int tub = idx;
@@ -239,7 +255,10 @@ struct Oracle {
                         idx++;
                         if (sum_in + sc >= (long)m * md((long)m + idx)) tub = idx;
                     }
-                    if (ub > tub) ub = tub;
+                    if (ub > tub) {
+                        ub = tub;
+                        st.lu_ub_tightened++;
+                    }
                     if (lb > 1) med = get_mindeg((long)m + lb);
                 }
             }
@@ -249,6 +268,7 @@ struct Oracle {
         }
         if (m + ub < ms) invalid = true;
         if (ub < 0 || ub < lb) invalid = true;
+        if (invalid) st.lu_invalid++;
         return invalid;
     }
 
@@ -258,7 +278,10 @@ struct Oracle {
         std::sort(cind.begin(), cind.begin() + nind, [](int a, int b) { return a > b; });
         if (lu_bounds(ub, lb, med, vs, m, nind, cind.data())) return true;
         for (int k = 0; k < m; k++)
-            if (!vert_ext(vs[k], m, ub, lb, med)) return true;
+            if (!vert_ext(vs[k], m, ub, lb, med)) {
+                st.member_fail++;
+                return true;
+            }
         int nrem = 0, nrmv = 0;
         for (int i = m; i < t; i++) {
             if (vs[i].label == 0 && cand_valid(vs[i], m, ub, lb, med))
@@ -268,6 +291,7 @@ struct Oracle {
         }
         while (nrem > 0 && nrmv > 0) {
             if (nrem < nrmv) {
+                st.dp_recompute++;
                 for (int i = 0; i < t; i++) vs[i].exdeg = 0;
                 for (int i = 0; i < nrem; i++) {
                     int v = vs[remaining[i]].id;
@@ -278,6 +302,7 @@ struct Oracle {
                     st.w_adj += roff[v + 1] - roff[v];
                 }
             } else {
+                st.dp_decrement++;
                 for (int i = 0; i < nrmv; i++) {
                     int v = vs[removed[i]].id;
                     for (uint64_t j = roff[v]; j < roff[v + 1]; j++) {
@@ -293,7 +318,10 @@ struct Oracle {
             std::sort(cind.begin(), cind.begin() + nv, [](int a, int b) { return a > b; });
             if (lu_bounds(ub, lb, med, vs, m, nv, cind.data())) return true;
             for (int k = 0; k < m; k++)
-                if (!vert_ext(vs[k], m, ub, lb, med)) return true;
+                if (!vert_ext(vs[k], m, ub, lb, med)) {
+                    st.member_fail++;
+                    return true;
+                }
             nv = 0;
             nrmv = 0;
             for (int k = 0; k < nrem; k++) {
@@ -399,13 +427,17 @@ struct Oracle {
                 for (int k = m; k < m + ncrit; k++)
                     if (adjc[k] < ncrit - 1) fail = true;
             if (fail) {
+                st.crit_fail++;
                 for (int i = 0; i < t; i++) munset(vs[i].id);
                 if (!faithful) full_reset();
                 return 2;
             }
+            st.crit_ok++;
             for (int k = m; k < m + ncrit; k++) vs[k].label = 1;
             m += ncrit;
             c -= ncrit;
+        } else {
+            st.crit_none++;
         }
         int nind = 0;
         for (int k = m; k < t; k++) {
@@ -503,7 +535,10 @@ struct Oracle {
         for (int j = cov; j < expansions; j++) {
             if (root && rb1 >= 0 && j - cov >= rb1) break;
             if (j != cov) {
-                if (remove_one_vertex(rec.data(), t, nc, nm)) break;
+                if (remove_one_vertex(rec.data(), t, nc, nm)) {
+                    st.remove_breaks++;
+                    break;
+                }
             }
             if (root && j - cov < b0) continue;
             vs.resize(t);
@@ -886,6 +921,18 @@ struct Oracle {
             st.max_depth = std::max(st.max_depth, w.st.max_depth);
             st.beyond += w.st.beyond;
             st.root_branches += w.st.root_branches;
+            st.remove_breaks += w.st.remove_breaks;
+            st.lu_low += w.st.lu_low;
+            st.lu_lb_invalid += w.st.lu_lb_invalid;
+            st.lu_sum_invalid += w.st.lu_sum_invalid;
+            st.lu_ub_tightened += w.st.lu_ub_tightened;
+            st.lu_invalid += w.st.lu_invalid;
+            st.member_fail += w.st.member_fail;
+            st.dp_recompute += w.st.dp_recompute;
+            st.dp_decrement += w.st.dp_decrement;
+            st.crit_none += w.st.crit_none;
+            st.crit_fail += w.st.crit_fail;
+            st.crit_ok += w.st.crit_ok;
             for (size_t i = 0; i + 1 < w.clq_off.size(); i++) {
                 clq_v.insert(clq_v.end(), w.clq_v.begin() + w.clq_off[i], w.clq_v.begin() + w.clq_off[i + 1]);
                 clq_off.push_back(clq_v.size());
@@ -904,7 +951,9 @@ extern "C" {
 
 struct orc_stats {
     uint64_t tasks, branches, children, raw_cliques, sum_t, sum_t_written, sum_clique_len, w_adj,
-        lookahead_hits, stale_oob, stale_hits, max_t, max_depth, root_size, init_w_adj, beyond, root_branches;
+        lookahead_hits, stale_oob, stale_hits, max_t, max_depth, root_size, init_w_adj, beyond, root_branches,
+        remove_breaks, lu_low, lu_lb_invalid, lu_sum_invalid, lu_ub_tightened, lu_invalid, member_fail,
+        dp_recompute, dp_decrement, crit_none, crit_fail, crit_ok;
 };
 
 struct orc_result {
@@ -974,6 +1023,27 @@ orc_result* orc_mine_window_mt(int n, const uint64_t* off, const int* adj, const
         }
     }
     return r;
+}
+
+// The root task record of initialize_tasks (main.cu:846-998) in its Q
+// order: ids, labels (0 candidate, 2 covered, 3 cover) and exdegs.  Returns
+// the record size; the arrays (n entries each) may be NULL.
+int orc_root(int n, const uint64_t* off, const int* adj, double gamma, int min_size, int* ids, int* labels,
+             int* exdeg) {
+    Oracle o;
+    o.n = n;
+    o.off = off;
+    o.adj = adj;
+    o.gamma = gamma;
+    o.ms = min_size;
+    o.build_tables();
+    std::vector<Vtx> root = o.build_root();
+    for (size_t i = 0; i < root.size(); i++) {
+        if (ids) ids[i] = root[i].id;
+        if (labels) labels[i] = root[i].label;
+        if (exdeg) exdeg[i] = root[i].exdeg;
+    }
+    return (int)root.size();
 }
 
 uint64_t orc_count(const orc_result* r) { return r->off.size() - 1; }

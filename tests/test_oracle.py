@@ -241,3 +241,107 @@ def test_window_from_the_end(name, build, gamma, ms):
     b = oracle.mine(off, adj, gamma, ms, root_begin=nb - k, root_end=-1)
     assert oracle.canon(a["raw"]) == oracle.canon(b["raw"])
     assert a["stats"] == b["stats"]
+
+
+# ---------------------------------------------------------------------------
+# root construction against a textbook k-core (initialize_tasks, main.cu:846-998)
+# ---------------------------------------------------------------------------
+def _textbook_root(off, adj, gamma, ms):
+    """The root task by definition: vertices with degree >= thr = mindeg[ms]
+    (and, below ms - 1 neighbours, at least ms - 1 vertices within two hops),
+    then the thr-core of what is left by repeated removal of any vertex with
+    fewer than thr neighbours left (the core is unique, so the order of
+    removals does not matter); the cover vertex is the core vertex with the
+    most core neighbours (lowest id among equals), its core neighbours are
+    covered; the record is sorted by label rank (covered, candidate, cover),
+    core degree, |N2| and id, the last three descending."""
+    n = len(off) - 1
+    nbr = [set(int(x) for x in adj[off[v]:off[v + 1]]) for v in range(n)]
+    thr = math.ceil(gamma * (ms - 1))
+    n2 = []
+    for v in range(n):
+        s = set(nbr[v])
+        for u in nbr[v]:
+            s |= nbr[u]
+        s.discard(v)
+        n2.append(len(s))
+    alive = {v for v in range(n) if len(nbr[v]) >= thr and (len(nbr[v]) >= ms - 1 or n2[v] >= ms - 1)}
+    changed = True
+    while changed:
+        changed = False
+        for v in sorted(alive):
+            if len(nbr[v] & alive) < thr:
+                alive.discard(v)
+                changed = True
+    if not alive:
+        return [], [], []
+    deg = {v: len(nbr[v] & alive) for v in alive}
+    best = max(deg.values())
+    cover = min(v for v in alive if deg[v] == best)
+    label = {v: 0 for v in alive}
+    label[cover] = 3
+    for u in nbr[cover] & alive:
+        label[u] = 2
+    rank = {2: 1, 0: 2, 3: 3}
+    order = sorted(alive, key=lambda v: (rank[label[v]], -deg[v], -n2[v], -v))
+    return order, [label[v] for v in order], [deg[v] for v in order]
+
+
+def _tie_graphs():
+    """Graphs whose cover choice and record order hinge on ties: equal
+    cliques, a clique with pendant paths, a ring of cliques."""
+    out = []
+    e = [(a, b) for base in (0, 6) for a in range(base, base + 6) for b in range(a + 1, base + 6)]
+    out.append(("two_k6", *csr_from_edges(12, e), 0.8, 4))
+    e = [(a, b) for a in range(7) for b in range(a + 1, 7)] + [(0, 7), (7, 8), (3, 9), (9, 10)]
+    out.append(("k7_paths", *csr_from_edges(11, e), 0.7, 3))
+    e = []
+    for k in range(4):
+        base = 5 * k
+        e += [(base + a, base + b) for a in range(5) for b in range(a + 1, 5)]
+        e.append((base + 4, (base + 5) % 20))
+    out.append(("ring_of_k5", *csr_from_edges(20, e), 0.75, 4))
+    return out
+
+
+@pytest.mark.parametrize("name,off,adj,gamma,ms",
+                         [(n, *b(), g, m) for n, b, g, m in SMALL_CASES] + _tie_graphs(),
+                         ids=[c[0] for c in SMALL_CASES] + [t[0] for t in _tie_graphs()])
+def test_root_matches_textbook_core(name, off, adj, gamma, ms):
+    r = oracle.root(off, adj, gamma, ms)
+    order, labels, deg = _textbook_root(off, adj, gamma, ms)
+    assert list(r["ids"]) == order, name
+    assert list(r["labels"]) == labels, name
+    assert list(r["exdeg"]) == deg, name
+
+
+def test_root_matches_textbook_core_config0():
+    from cuqc_amd import Graph
+    g = Graph.generate(0, 1)
+    off, adj = g.csr()
+    r = oracle.root(off, adj, g.gamma, g.min_size)
+    order, labels, deg = _textbook_root(off, adj, g.gamma, g.min_size)
+    assert list(r["ids"]) == order and list(r["labels"]) == labels and list(r["exdeg"]) == deg
+
+
+# ---------------------------------------------------------------------------
+# every pruning rule fires on the inputs the GPU parity tests mine
+# ---------------------------------------------------------------------------
+RULES = ["lookahead_hits", "remove_breaks", "lu_low", "lu_lb_invalid", "lu_sum_invalid", "lu_ub_tightened",
+         "lu_invalid", "member_fail", "dp_recompute", "dp_decrement", "crit_none", "crit_fail", "crit_ok"]
+
+
+def test_every_rule_fires_in_the_gpu_parity_set():
+    """Workload 0 and the small cases are mined on the GPU and compared with
+    the oracle counter for counter (tests/test_gpu_parity.py); here the
+    oracle shows that each rule of the reference's pruning takes effect on
+    them, so a GPU that skipped or mis-stated one would change a count."""
+    from cuqc_amd import Graph
+    g = Graph.generate(0, 1)
+    cases = [(*g.csr(), g.gamma, g.min_size)] + [(*b(), gm, ms) for _, b, gm, ms in SMALL_CASES]
+    tot = {k: 0 for k in RULES}
+    for off, adj, gm, ms in cases:
+        st = oracle.mine(off, adj, gm, ms)["stats"]
+        for k in RULES:
+            tot[k] += st[k]
+    assert all(tot[k] > 0 for k in RULES), tot
