@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
         L.orc_nonmax.argtypes = [C.c_uint64, P, P]
         L.orc_build_n2.restype = C.c_uint64
         L.orc_build_n2.argtypes = [C.c_int, P, P, P, P]
+        L.orc_lu_bounds.restype = None
+        L.orc_lu_bounds.argtypes = [C.c_double, C.c_int, C.c_int, P, P, C.c_int, P, P]
         L.orc_root.restype = C.c_int
         L.orc_root.argtypes = [C.c_int, P, P, C.c_double, C.c_int, P, P, P]
         L.orc_brute.restype = P
@@ -170,6 +172,20 @@ def root(off, adj, gamma: float, min_size: int) -> dict:
     k = L.orc_root(n, off.ctypes.data, adj.ctypes.data if adj.size else None, float(gamma), int(min_size),
                    ids.ctypes.data, labels.ctypes.data, ex.ctypes.data)
     return {"ids": ids[:k].copy(), "labels": labels[:k].copy(), "exdeg": ex[:k].copy()}
+
+
+def lu_bounds(gamma: float, min_size: int, members, cand_indeg) -> tuple[bool, int, int, int]:
+    """The oracle's LU bounds (main.cu:1857-1972) for members given as
+    (indeg, exdeg) pairs and candidate indegrees (sorted descending here):
+    (invalid, lb, ub, min_ext_deg)."""
+    L = lib()
+    ind = np.array([x[0] for x in members], np.int32)
+    exd = np.array([x[1] for x in members], np.int32)
+    ci = np.array(sorted(cand_indeg, reverse=True) or [0], np.int32)
+    out = np.zeros(4, np.int32)
+    L.orc_lu_bounds(float(gamma), int(min_size), len(members), ind.ctypes.data, exd.ctypes.data, len(cand_indeg),
+                    ci.ctypes.data, out.ctypes.data)
+    return bool(out[0]), int(out[1]), int(out[2]), int(out[3])
 
 
 def canon(sets) -> list[tuple[int, ...]]:

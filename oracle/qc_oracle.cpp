@@ -1046,6 +1046,25 @@ int orc_root(int n, const uint64_t* off, const int* adj, double gamma, int min_s
     return (int)root.size();
 }
 
+// lu_bounds (main.cu:1857-1972) on a given record: members' (indeg,
+// exdeg) and the candidates' indegrees sorted descending.  out = {invalid,
+// lb, ub, min_ext_deg}.
+void orc_lu_bounds(double gamma, int min_size, int m, const int* indeg, const int* exdeg, int nc, const int* cind,
+                   int* out) {
+    Oracle o;
+    o.n = 2 * (m + nc) + 16;
+    o.gamma = gamma;
+    o.ms = min_size;
+    o.build_tables();
+    std::vector<Vtx> vs(m);
+    for (int i = 0; i < m; i++) vs[i] = Vtx{i, 1, indeg[i], exdeg[i], 0};
+    int ub = 0, lb = 0, med = 0;
+    out[0] = o.lu_bounds(ub, lb, med, vs.data(), m, nc, cind) ? 1 : 0;
+    out[1] = lb;
+    out[2] = ub;
+    out[3] = med;
+}
+
 uint64_t orc_count(const orc_result* r) { return r->off.size() - 1; }
 const uint64_t* orc_off(const orc_result* r) { return r->off.data(); }
 const int* orc_verts(const orc_result* r) { return r->v.data(); }

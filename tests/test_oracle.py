@@ -345,3 +345,37 @@ def test_every_rule_fires_in_the_gpu_parity_set():
         for k in RULES:
             tot[k] += st[k]
     assert all(tot[k] > 0 for k in RULES), tot
+
+
+# ---------------------------------------------------------------------------
+# LU bounds: cases traced by hand through main.cu:1857-1972
+# ---------------------------------------------------------------------------
+def test_lu_bounds_hand_traced():
+    """mindeg[i] = ceil(gamma (i - 1)); get_mindeg(s) = mindeg[max(s, ms)].
+
+    A. gamma 0.9, ms 5, members (1,5) (1,3), candidate indegrees 2 2 1 0.
+       min indeg 1 is not below mindeg[2] = 1, so ub = c = 4 and
+       lb = ms - m = 3; min_ext_deg = get_mindeg(3) = mindeg[5] = 4.
+    B. gamma 0.9, ms 5, members (1,4) (2,2) (1,6), candidates 3 2 2 1 0.
+       min indeg 1 < mindeg[3] = 2: lb starts at mindeg[5] - 1 = 3 and grows
+       while lb <= 4 (the smallest exdeg at indeg 1) and 1 + lb <
+       mindeg[3 + lb]: 4 < 5, 5 < 6, stop at lb = 5 > 4; 6 < mindeg[8] = 7
+       still fails, so lb = c + 1 = 6 and the bounds are invalid;
+       ub = floor(4 / 0.9) + 1 - 3 = 2.
+    C. gamma 0.9, ms 5, members (1,9) (2,8) (2,9), candidates
+       3 3 3 2 2 2 1 1 0 0.  lb climbs 3 -> 8 (9 < mindeg[11] = 9 fails);
+       ub = floor(10 / 0.9) + 1 - 3 = 9.  The sum test: 5 + 17 = 22 <
+       3 mindeg[11] = 27 takes idx to 9 (adding 0), and 22 < 3 mindeg[12] =
+       30 still holds: lb = ub + 1 = 10, invalid.
+    D. gamma 0.5, ms 3, members (1,5) (3,3) (2,4) (3,2), candidates
+       3 2 2 1 0 0.  min indeg 1 < mindeg[4] = 2: lb = 2 - 1 = 1 (2 <
+       mindeg[5] = 2 fails at once); ub = floor(5 / 0.5) + 1 - 4 = 7, cut to
+       c = 6.  The sum test passes at idx 1 (9 + 3 >= 4 mindeg[5] = 8); the
+       running sums 14, 16, 17, 17, 17 against 4 mindeg[6..10] = 12, 12, 16,
+       16, 20 keep tub = 5, so ub = 5; lb = 1 leaves min_ext_deg at
+       get_mindeg(5) = 2.
+    """
+    assert oracle.lu_bounds(0.9, 5, [(1, 5), (1, 3)], [2, 2, 1, 0]) == (False, 3, 4, 4)
+    assert oracle.lu_bounds(0.9, 5, [(1, 4), (2, 2), (1, 6)], [3, 2, 2, 1, 0]) == (True, 6, 2, 4)
+    assert oracle.lu_bounds(0.9, 5, [(1, 9), (2, 8), (2, 9)], [3, 3, 3, 2, 2, 2, 1, 1, 0, 0]) == (True, 10, 9, 4)
+    assert oracle.lu_bounds(0.5, 3, [(1, 5), (3, 3), (2, 4), (3, 2)], [3, 2, 2, 1, 0, 0]) == (False, 1, 5, 2)
