@@ -32,6 +32,8 @@ HEADERS = ["qc_host.h", "qc_device.h", "qc_internal.h"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # pool blocks hold a free-list link in the bytes of a task record
+    "-fno-strict-aliasing",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",

@@ -12,6 +12,8 @@
 // (CLIQUES_DUMP, main.cu:56).
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cstring>
 
@@ -174,6 +176,11 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
         return QC_OK;
     };
 
+    // QCMINE_DEBUG=1: the reference's per-level size report on stdout
+    const bool debug = getenv("QCMINE_DEBUG") && atoi(getenv("QCMINE_DEBUG")) > 0;
+    struct {
+        uint64_t tasks = 0, buf = 0, buf_off = 0, clq = 0, clq_off = 0, child = 0, child_off = 0;
+    } mx;
     std::vector<TaskRef> batch, kids;
     while (!st[0].refs.empty() || !st[1].refs.empty()) {
         const int k = st[1].refs.empty() ? 0 : 1;
@@ -277,9 +284,40 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
             }
             D.top += oc[j].slots;
         }
+        if (debug) {
+            // the reference's debug size report (main.cu:2635-2646): level,
+            // tasks expanded (records, vertex slots), children written,
+            // records waiting on the host stacks, raw cliques held
+            StreamCtr cc;
+            QC_HIP(hipMemcpyAsync(&cc, d_clq_ctr.p, sizeof(cc), hipMemcpyDeviceToHost, s));
+            QC_HIP(hipStreamSynchronize(s));
+            uint64_t bslots = 0;
+            for (const TaskRef& r : batch) bslots += r.t;
+            const uint64_t wrefs = st[0].refs.size() + st[1].refs.size(), wslots = st[0].top + st[1].top;
+            printf("L: %llu T1: %llu %llu T2: %llu %llu B: %llu %llu C: %llu %llu\n\n", (unsigned long long)out.levels,
+                   (unsigned long long)take, (unsigned long long)bslots,
+                   (unsigned long long)(oc[0].items + oc[1].items), (unsigned long long)(oc[0].slots + oc[1].slots),
+                   (unsigned long long)wrefs, (unsigned long long)wslots, cc.items, cc.slots);
+            mx.tasks = std::max<uint64_t>(mx.tasks, bslots);
+            mx.buf = std::max<uint64_t>(mx.buf, wslots);
+            mx.buf_off = std::max<uint64_t>(mx.buf_off, wrefs);
+            mx.clq = std::max<uint64_t>(mx.clq, cc.slots);
+            mx.clq_off = std::max<uint64_t>(mx.clq_off, cc.items);
+            mx.child = std::max<uint64_t>(mx.child, oc[0].slots + oc[1].slots);
+            mx.child_off = std::max<uint64_t>(mx.child_off, oc[0].items + oc[1].items);
+        }
         if ((rc = drain(false))) return rc;
     }
     if ((rc = drain(true))) return rc;
+    if (debug) {
+        // print_maxes (main.cu:2893-2907): the largest of each buffer over the run
+        printf("\nTASKS SIZE: %llu\nBUFFER SIZE: %llu\nBUFFER OFFSET SIZE: %llu\nCLIQUES SIZE: %llu\n"
+               "CLIQUES OFFSET SIZE: %llu\nWTASKS SIZE: %llu\nWTASKS OFFSET SIZE: %llu\n\n",
+               (unsigned long long)mx.tasks, (unsigned long long)mx.buf, (unsigned long long)mx.buf_off,
+               (unsigned long long)mx.clq, (unsigned long long)mx.clq_off, (unsigned long long)mx.child,
+               (unsigned long long)mx.child_off);
+        fflush(stdout);
+    }
     QC_HIP(hipMemcpyAsync(&out.ds, d_stats.p, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     QC_HIP(hipStreamSynchronize(s));
     (void)hipEventDestroy(e0);

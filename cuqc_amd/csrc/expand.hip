@@ -1043,13 +1043,31 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
 // ---------------------------------------------------------------------------
 constexpr unsigned long long FL_IDX = (1ull << 36) - 1;  // block index field; all ones = empty list
 
-// A block of the record's size class (lane 0): the class's free list first,
-// else fresh units from the bump pointer.  Returns the Vrec offset, or ~0
-// when the pool is exhausted (the bump pointer keeps counting: the host
-// reads what the launch needed).
+// Free blocks of the small classes (up to 8 units) first go on a free list
+// of the wave that frees them (heads in wave_fl, only that wave's lane 0
+// touches them: no atomics); up to 64 units per class stay there, the rest
+// and every larger block go on the class's shared lock-free list.  Tiny
+// tasks free and allocate a block each, and one contended list head would
+// serialize them.
+constexpr uint32_t FL_LOCAL_UNITS = 64;
+
+// A block of the record's size class (lane 0): this wave's own list, the
+// class's shared list, else fresh units from the bump pointer.  Returns the
+// Vrec offset, or ~0 when the pool is exhausted (the bump pointer keeps
+// counting: the host reads what the launch needed).
 __device__ unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
     uint32_t units = 0;
     const int cl = pool_class(nvrec, units);
+    if (units <= 8) {
+        unsigned long long* lp = a.wave_fl + (size_t)blockIdx.x * POOL_NCLASS + cl;
+        const unsigned long long lh = *lp;
+        if ((lh & FL_IDX) != FL_IDX) {
+            const unsigned long long idx = lh & FL_IDX;
+            const unsigned long long nxt = *(const unsigned long long*)(a.pool + idx * 64);
+            *lp = (((lh >> 36) - 1ull) << 36) | (nxt & FL_IDX);
+            return idx * 64;
+        }
+    }
     unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
     unsigned long long h = __hip_atomic_load(hp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     while ((h & FL_IDX) != FL_IDX) {
@@ -1069,13 +1087,23 @@ __device__ unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
     return off + n <= a.pool_cap ? off : ~0ull;
 }
 
-// Gives a block back to its class's free list (lane 0, after the wave has
-// read the record: the release orders those reads before any reuse).
+// Gives a block back (lane 0, after the wave has read the record: the
+// release of a shared push orders those reads before any reuse).
 __device__ void pool_free(const ExpandArgs& a, unsigned long long off, uint32_t nvrec) {
     uint32_t units = 0;
     const int cl = pool_class(nvrec, units);
-    unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
     const unsigned long long idx = off / 64;
+    if (units <= 8) {
+        unsigned long long* lp = a.wave_fl + (size_t)blockIdx.x * POOL_NCLASS + cl;
+        const unsigned long long lh = *lp;
+        const unsigned long long cnt = (lh & FL_IDX) == FL_IDX ? 0ull : lh >> 36;
+        if ((cnt + 1) * units <= FL_LOCAL_UNITS) {
+            *(unsigned long long*)(a.pool + off) = lh & FL_IDX;
+            *lp = ((cnt + 1ull) << 36) | idx;
+            return;
+        }
+    }
+    unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
     unsigned long long h = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         __hip_atomic_store((unsigned long long*)(a.pool + off), h & FL_IDX, __ATOMIC_RELAXED,
@@ -1967,7 +1995,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             idle = 0;
             if (!a.static_sched && c.ptop - c.pbot > 1) {
                 // the rings run low while others wait: hand over the oldest
-                // (shallowest, so largest) records, up to 4 per task
+                // (shallowest, so largest) records, up to 16 at once, into
+                // the bucket of the oldest one
                 unsigned long long shared = 0;
                 if (c.lane == 0)
                     for (int b = 0; b < a.n_buckets; b++) {
@@ -1978,20 +2007,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                         shared += tb > hb ? tb - hb : 0;
                     }
                 shared = rdlane64(shared, 0);
-                for (int d = 0; d < 4 && shared + d < G / 4 && c.ptop - c.pbot > 1; d++) {
+                if (shared < G / 4) {
+                    const int want = (int)min<unsigned long long>(
+                        min<unsigned long long>((unsigned long long)(c.ptop - c.pbot - 1), 16ull), G / 4 - shared);
                     int bk = 0;
-                    TaskRef r;
-                    if (c.lane == 0) {
-                        r = c.pstack[c.pbot % a.priv_cap];
-                        bk = c.pbkt[c.pbot % a.priv_cap];
-                    }
-                    r = bcast_ref(r, 0);
+                    if (c.lane == 0) bk = c.pbkt[c.pbot % a.priv_cap];
                     bk = rdlane(bk, 0);
                     unsigned long long pos = 0;
-                    if (ring_claim(c, bk, 1, &pos) != 1) break;
-                    if (c.lane == 0) a.q_refs[(unsigned long long)bk * a.q_cap + pos % a.q_cap] = r;
-                    ring_publish(c, bk, pos, 1);
-                    c.pbot++;
+                    const int k = ring_claim(c, bk, want, &pos);
+                    if (k > 0) {
+                        if (c.lane == 0)
+                            for (int j = 0; j < k; j++)
+                                a.q_refs[(unsigned long long)bk * a.q_cap + (pos + j) % a.q_cap] =
+                                    c.pstack[(c.pbot + (uint32_t)j) % a.priv_cap];
+                        ring_publish(c, bk, pos, k);
+                        c.pbot += (uint32_t)k;
+                    }
                 }
             }
             c.ptop--;

@@ -185,8 +185,27 @@ __global__ void nm_scatter(const uint64_t* __restrict__ off, const uint32_t* __r
     }
 }
 
+// sort keys of the postings: the complement of the posted set's size (so
+// an ascending sort puts the largest sets first); past the postings ~0
+__global__ void nm_post_keys(const uint64_t* __restrict__ off, const uint32_t* __restrict__ post,
+                             const uint32_t* __restrict__ start, const uint32_t* __restrict__ cnt, uint32_t nkeys,
+                             uint64_t nv, uint32_t* __restrict__ pkey, uint32_t* __restrict__ pend) {
+    const uint64_t npost = (uint64_t)start[nkeys - 1] + cnt[nkeys - 1];
+    const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nv; q += nt) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (q < npost) {
+            const uint32_t t = post[q];
+            key = ~(uint32_t)(off[t + 1] - off[t]);
+        }
+        pkey[q] = key;
+    }
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nkeys; x += nt) pend[x] = start[x] + cnt[x];
+}
+
 // maximal[u] = 1 iff unique set uniq[u] (u < nu) is not a proper subset of
-// another; 0 for nu <= u < k
+// another; 0 for nu <= u < k.  Postings list the largest sets first: the
+// scan stops at the first superset, or at the first set no longer than s.
 __global__ __launch_bounds__(64) void nm_subset(const uint64_t* __restrict__ off, const uint32_t* __restrict__ v,
                                                 const uint32_t* __restrict__ uniq, const uint32_t* __restrict__ nu_p,
                                                 uint64_t k, const uint32_t* __restrict__ start,
@@ -223,7 +242,7 @@ __global__ __launch_bounds__(64) void nm_subset(const uint64_t* __restrict__ off
         for (uint32_t q = pb; q < pe && !sub; q++) {
             const uint32_t t = post[q];
             const uint64_t tb = off[t], te = off[t + 1];
-            if (t == s || te - tb <= ls) continue;
+            if (te - tb <= ls) break;
             bool miss = false;
             for (uint64_t j = sb + lane; j < se; j += 64) {
                 uint32_t x = v[j];
@@ -323,8 +342,8 @@ namespace {
 
 // Byte layout of the scratch region for k sets, nv vertices and nkeys keys.
 struct NmPlan {
-    size_t o, lens, oo, keys_in, keys, inst, h, hs, idx, ids, keep, pos, uniq, cnt, start, cur, post, mx, mpos, mlist,
-        sorted, ctr, temp, temp_bytes, total;
+    size_t o, lens, oo, keys_in, keys, inst, h, hs, idx, ids, keep, pos, uniq, cnt, start, cur, post, pkey, pkey2,
+        post2, mx, mpos, mlist, sorted, ctr, temp, temp_bytes, total;
 };
 
 size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -346,6 +365,7 @@ int nm_plan(uint64_t k, uint64_t nv, uint32_t nkeys, NmPlan& P, hipStream_t s) {
     NM_Q(rocprim::radix_sort_pairs(nullptr, t, u64, u64, u32, u32, (size_t)k, 0, 64, s));
     NM_Q(rocprim::exclusive_scan(nullptr, t, u32, u32, 0u, (size_t)k, rocprim::plus<uint32_t>(), s));
     NM_Q(rocprim::exclusive_scan(nullptr, t, u32, u32, 0u, (size_t)nkeys, rocprim::plus<uint32_t>(), s));
+    NM_Q(rocprim::segmented_radix_sort_pairs(nullptr, t, u32, u32, u32, u32, nvu, nkeys, u32, u32 + 1, 0, 32, s));
     qcdev::LexLess less{nullptr, nullptr};
     NM_Q(rocprim::merge_sort(nullptr, t, u32, u32, (size_t)k, less, s));
 #undef NM_Q
@@ -374,6 +394,9 @@ int nm_plan(uint64_t k, uint64_t nv, uint32_t nkeys, NmPlan& P, hipStream_t s) {
     P.cnt = take((size_t)nkeys * 4);
     P.start = take((size_t)nkeys * 4);
     P.cur = take((size_t)nkeys * 4);
+    P.pkey = take(nv * 4);
+    P.pkey2 = take(nv * 4);
+    P.post2 = take(nv * 4);
     P.mx = take(k * 4);
     P.mpos = take(k * 4);
     P.ctr = take(16);
@@ -467,10 +490,19 @@ int nonmax_run(const NmInput& in, char* scr, size_t scr_bytes, const NmOutput& o
     hipLaunchKernelGGL(qcdev::nm_scatter, dim3(gw), dim3(64), 0, s, off, keys, U32(P.uniq), nu_p, U32(P.start),
                        U32(P.cur), U32(P.post));
     QC_HIP(hipGetLastError());
+    // every postings list: largest sets first (the cursor array is free
+    // again and takes the segment ends)
+    const unsigned gv = (unsigned)std::min<uint64_t>((std::max<uint64_t>(nv, nkeys) + 255) / 256, 16384);
+    hipLaunchKernelGGL(qcdev::nm_post_keys, dim3(gv), dim3(256), 0, s, off, U32(P.post), U32(P.start), U32(P.cnt),
+                       nkeys, nv, U32(P.pkey), U32(P.cur));
+    QC_HIP(hipGetLastError());
+    tb = P.temp_bytes;
+    QC_RP(rocprim::segmented_radix_sort_pairs(temp, tb, U32(P.pkey), U32(P.pkey2), U32(P.post), U32(P.post2),
+                                             (unsigned int)nv, nkeys, U32(P.start), U32(P.cur), 0, 32, s));
 
     // 4. subset test
     hipLaunchKernelGGL(qcdev::nm_subset, dim3(gw), dim3(64), 0, s, off, keys, U32(P.uniq), nu_p, k, U32(P.start),
-                       U32(P.cnt), U32(P.post), U32(P.mx));
+                       U32(P.cnt), U32(P.post2), U32(P.mx));
     QC_HIP(hipGetLastError());
     tb = P.temp_bytes;
     QC_RP(rocprim::exclusive_scan(temp, tb, U32(P.mx), U32(P.mpos), 0u, (size_t)k, rocprim::plus<uint32_t>(), s));

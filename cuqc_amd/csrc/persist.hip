@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -30,7 +31,8 @@ __global__ void expand_persistent(const ExpandArgs*);
 // the pool; allocated blocks start at the next 64-Vrec boundary.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
                               uint64_t root_slots, TaskRef* q_refs, unsigned long long* q_seq, uint64_t q_cap, int nb,
-                              Vrec* pool, unsigned long long* ctr, unsigned long long* fl_head, StreamCtr* clq_ctr,
+                              Vrec* pool, unsigned long long* ctr, unsigned long long* fl_head,
+                              unsigned long long* wave_fl, uint32_t n_waves, StreamCtr* clq_ctr,
                               DevStats* stats, DevStats* inst_stats, uint32_t n_inst, int* error,
                               const float* __restrict__ root_feat, float* trace_feat, long long* trace_parent,
                               unsigned int* trace_beyond) {
@@ -49,6 +51,7 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
     }
     for (uint64_t i = tid; i < root_slots; i += nt) pool[i] = root_verts[i];
     for (uint64_t i = tid; i < (uint64_t)POOL_NCLASS; i += nt) fl_head[i * QSTRIDE] = (1ull << 36) - 1;
+    for (uint64_t i = tid; i < (uint64_t)n_waves * POOL_NCLASS; i += nt) wave_fl[i] = (1ull << 36) - 1;
     for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (tid == 0) {
         for (int b = 0; b < nb; b++) {
@@ -90,7 +93,7 @@ struct RunState {
 // serialize on the mutex.  Never destroyed: the runtime may be gone at exit.
 struct Workspace {
     std::mutex mu;
-    DBuf pool, q, seq, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args, fl, priv, pbkt;
+    DBuf pool, q, seq, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args, fl, wfl, priv, pbkt;
     uint64_t pool_cap = 0, q_cap = 0, clq_slots = 0, clq_refs_cap = 0, priv_cap = 0;
 
     static Workspace& get(int dev) {
@@ -294,7 +297,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         if ((rc = ensure(ws.pool, pool_cap * sizeof(Vrec))) || (rc = ensure(ws.q, nb * q_cap * sizeof(TaskRef))) ||
             (rc = ensure(ws.seq, nb * q_cap * 8)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
             (rc = ensure(ws.clq_verts, clq_slots * 4)) || (rc = ensure(ws.priv, (size_t)grid * priv_cap * sizeof(TaskRef))) ||
-            (rc = ensure(ws.pbkt, (size_t)grid * priv_cap)))
+            (rc = ensure(ws.pbkt, (size_t)grid * priv_cap)) ||
+            (rc = ensure(ws.wfl, (size_t)grid * POOL_NCLASS * 8)))
             return rc;
         if (trace && ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8)) ||
                       (rc = d_tbey.alloc(q_cap * 4))))
@@ -332,6 +336,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.q_refs = ws.q.as<TaskRef>();
         A.q_seq = ws.seq.as<unsigned long long>();
         A.fl_head = ws.fl.as<unsigned long long>();
+        A.wave_fl = ws.wfl.as<unsigned long long>();
         A.priv = ws.priv.as<TaskRef>();
         A.priv_bkt = ws.pbkt.as<unsigned char>();
         A.priv_cap = (uint32_t)priv_cap;
@@ -372,7 +377,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
                            d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), ws.q.as<TaskRef>(),
                            ws.seq.as<unsigned long long>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
-                           ws.fl.as<unsigned long long>(), ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
+                           ws.fl.as<unsigned long long>(), ws.wfl.as<unsigned long long>(), grid,
+                           ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
                            (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent,
                            A.trace_beyond);
         hipError_t le = hipLaunchKernel(kern, dim3(grid), dim3(64), kargs, lds_b, s);
@@ -522,6 +528,27 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         outs[i].kernel_ms = first_ms;
         outs[i].levels = 1;
         outs[i].pool_peak = first.ctr[2 * nb];
+    }
+    if (getenv("QCMINE_DEBUG") && atoi(getenv("QCMINE_DEBUG")) > 0) {
+        // the reference's size report (main.cu:2635-2646, 2893-2907) for the
+        // one persistent launch: tasks expanded, children written, the pool's
+        // high-water mark (B: records are not counted, slots are) and the raw
+        // cliques, then the largest buffers
+        DevStats t{};
+        for (const DevStats& d : first_stats) {
+            t.tasks += d.tasks;
+            t.sum_t += d.sum_t;
+            t.children += d.children;
+            t.sum_t_written += d.sum_t_written;
+            t.raw_cliques += d.raw_cliques;
+            t.sum_clique_len += d.sum_clique_len;
+        }
+        printf("L: 1 T1: %llu %llu T2: %llu %llu B: 0 %llu C: %llu %llu\n\n", t.tasks, t.sum_t, t.children,
+               t.sum_t_written, (unsigned long long)first.ctr[2 * nb], t.raw_cliques, t.sum_clique_len);
+        printf("\nTASKS SIZE: %llu\nBUFFER SIZE: %llu\nCLIQUES SIZE: %llu\nCLIQUES OFFSET SIZE: %llu\n\n",
+               (unsigned long long)pool_cap, (unsigned long long)first.ctr[2 * nb],
+               (unsigned long long)first.clq.slots, (unsigned long long)first.clq.items);
+        fflush(stdout);
     }
     // the ENUMERATION window ends with the raw sets collected; RemoveNonMax
     // then runs on the raw sets where the kernel left them, all instances in

@@ -184,6 +184,8 @@ struct ExpandArgs {
     unsigned long long pool_cap;
     unsigned long long* fl_head;     // free-list heads, POOL_NCLASS lines of QSTRIDE words:
                                      // (tag << 36) | block index (64-Vrec units), all ones = empty
+    unsigned long long* wave_fl;     // per wave, POOL_NCLASS heads of its own free lists:
+                                     // (count << 36) | block index, index all ones = empty
     long long* pending;
     unsigned long long* overflow;    // pushes that found the ring and the private stack full
     // private stacks: wave w owns priv[w priv_cap, (w+1) priv_cap) (refs)

@@ -42,6 +42,9 @@ def lib() -> C.CDLL:
         L.orc_mine_window_mt.restype = P
         L.orc_mine_window_mt.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.c_long,
                                          C.c_long, C.c_int, C.c_int]
+        L.orc_mine_window2.restype = P
+        L.orc_mine_window2.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.c_long,
+                                       C.c_long, C.c_long, C.c_long, C.c_int, C.c_int]
         L.orc_count.restype = C.c_uint64
         L.orc_count.argtypes = [P]
         L.orc_off.restype = C.POINTER(C.c_uint64)
@@ -89,10 +92,12 @@ def _csr(off, adj):
 
 
 def mine(off, adj, gamma: float, min_size: int, *, faithful: bool = False, trace: bool = False, off2=None,
-         adj2=None, root_begin: int = 0, root_end: int = -1, max_depth: int = -1, threads: int = 1) -> dict:
+         adj2=None, root_begin: int = 0, root_end: int = -1, max_depth: int = -1, threads: int = 1,
+         l1_begin: int = 0, l1_end: int = -1) -> dict:
     """Run the restated host path to completion (or over a search window:
-    root branches [root_begin, root_end) and depth <= max_depth); returns raw
-    sets, stats and (optionally) the per-task trace.  threads > 1 runs the
+    root branches [root_begin, root_end), branches [l1_begin, l1_end) of every
+    depth-1 task and depth <= max_depth); returns raw sets, stats and
+    (optionally) the per-task trace.  threads > 1 runs the
     same search on that many host threads (clean map mode, no trace): equal
     counters and raw multiset, raw sets in another order."""
     if threads > 1 and (faithful or trace):
@@ -103,9 +108,9 @@ def mine(off, adj, gamma: float, min_size: int, *, faithful: bool = False, trace
     if off2 is not None:
         off2, adj2 = _csr(off2, adj2)
         o2, a2 = off2.ctypes.data, adj2.ctypes.data
-    h = L.orc_mine_window_mt(len(off) - 1, off.ctypes.data, adj.ctypes.data if adj.size else None, o2, a2,
-                             float(gamma), int(min_size), int(faithful), int(trace), int(root_begin), int(root_end),
-                             int(max_depth), int(threads))
+    h = L.orc_mine_window2(len(off) - 1, off.ctypes.data, adj.ctypes.data if adj.size else None, o2, a2,
+                           float(gamma), int(min_size), int(faithful), int(trace), int(root_begin), int(root_end),
+                           int(l1_begin), int(l1_end), int(max_depth), int(threads))
     try:
         st = _Stats()
         L.orc_get_stats(h, C.byref(st))

@@ -139,6 +139,8 @@ struct Oracle {
     // reference's loop order; rb0 < 0 starts that many branches before the
     // end) and tasks of depth <= max_depth are expanded
     long rb0 = 0, rb1 = -1;
+    // and the branches [lb0, lb1) of every depth-1 task (same conventions)
+    long lb0 = 0, lb1 = -1;
     int max_depth = -1;
     int cur_depth = 0;
 
@@ -530,17 +532,20 @@ struct Oracle {
         std::vector<Vtx> vs;
         const bool root = cur_depth == 0;
         if (root) st.root_branches = expansions - cov;
-        // a negative window start counts from the end of the root's branches
-        const long b0 = rb0 >= 0 ? rb0 : std::max(0L, (long)(expansions - cov) + rb0);
+        // the window of this task's branches: the root's, a depth-1 task's,
+        // or all; a negative start counts from the end of the branches
+        const bool win = cur_depth <= 1;
+        const long wb0 = cur_depth == 0 ? rb0 : lb0, wb1 = cur_depth == 0 ? rb1 : lb1;
+        const long b0 = !win ? 0 : wb0 >= 0 ? wb0 : std::max(0L, (long)(expansions - cov) + wb0);
         for (int j = cov; j < expansions; j++) {
-            if (root && rb1 >= 0 && j - cov >= rb1) break;
+            if (win && wb1 >= 0 && j - cov >= wb1) break;
             if (j != cov) {
                 if (remove_one_vertex(rec.data(), t, nc, nm)) {
                     st.remove_breaks++;
                     break;
                 }
             }
-            if (root && j - cov < b0) continue;
+            if (j - cov < b0) continue;
             vs.resize(t);
             int m = nm, c = nc, tt = t;
             for (int i = 0; i < m; i++) vs[i] = rec[i];
@@ -852,6 +857,8 @@ struct Oracle {
             w.mindeg = mindeg;
             w.rb0 = rb0;
             w.rb1 = rb1;
+            w.lb0 = lb0;
+            w.lb1 = lb1;
             w.max_depth = max_depth;
             w.pos.assign(n, -1);
             w.remaining.assign(n + 1, 0);
@@ -971,6 +978,9 @@ orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const ui
 orc_result* orc_mine_window_mt(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                                double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth,
                                int threads);
+orc_result* orc_mine_window2(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
+                             double gamma, int min_size, int faithful, int trace, long rb0, long rb1, long lb0,
+                             long lb1, int max_depth, int threads);
 
 orc_result* orc_mine(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                      double gamma, int min_size, int faithful, int trace) {
@@ -988,6 +998,14 @@ orc_result* orc_mine_window(int n, const uint64_t* off, const int* adj, const ui
 orc_result* orc_mine_window_mt(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
                                double gamma, int min_size, int faithful, int trace, long rb0, long rb1, int max_depth,
                                int threads) {
+    return orc_mine_window2(n, off, adj, off2, adj2, gamma, min_size, faithful, trace, rb0, rb1, 0, -1, max_depth,
+                            threads);
+}
+
+// ... and the branches [lb0, lb1) of every depth-1 task
+orc_result* orc_mine_window2(int n, const uint64_t* off, const int* adj, const uint64_t* off2, const int* adj2,
+                             double gamma, int min_size, int faithful, int trace, long rb0, long rb1, long lb0,
+                             long lb1, int max_depth, int threads) {
     if (threads > 1 && (faithful || trace)) return nullptr;
     Oracle o;
     o.n = n;
@@ -1001,6 +1019,8 @@ orc_result* orc_mine_window_mt(int n, const uint64_t* off, const int* adj, const
     o.trace = trace != 0;
     o.rb0 = rb0;
     o.rb1 = rb1;
+    o.lb0 = lb0;
+    o.lb1 = lb1;
     o.max_depth = max_depth;
     o.build_tables();
     if (threads > 1)

@@ -84,12 +84,13 @@ def _window_part(args):
     the run over the empty window [b0, b0), which holds the root's own work
     (its task, lookahead and the removals that lead up to branch b0); the
     first part of a window also returns that root share."""
-    wl, inst, b0, b1, first = args
+    wl, inst, b0, b1, first, threads = args
     from cuqc_amd import workloads
     w = workloads.WORKLOADS[wl]
     g = workloads.graph(w, inst)
     off, adj = g.csr()
-    r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=w.max_depth)
+    r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=w.max_depth,
+                    threads=threads)
     e = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b0, max_depth=w.max_depth)
     st = {k: r["stats"][k] - e["stats"][k] for k in STATS}
     return st, [[int(x) for x in s] for s in r["raw"]], e["stats"] if first else None
@@ -105,11 +106,15 @@ def window_result(wl: int, inst: int = 0, jobs: int = 8, parts: int = 64) -> dic
     b0 = w.root_begin if w.root_begin >= 0 else max(0, nb + w.root_begin)
     b1 = nb if w.root_end < 0 else min(nb, w.root_end)
     cuts = sorted(set([b0] + [b0 + (b1 - b0) * i // parts for i in range(1, parts)] + [b1]))
+    # fewer parts than workers (a window of a few deep root branches): each
+    # part runs the oracle on several threads
+    jobs = int(os.environ.get("GOLDEN_JOBS", jobs))
+    threads = max(1, jobs // max(1, len(cuts) - 1))
     # the first part also returns the root's own counters
-    tasks = [(wl, inst, a, b, a == b0) for a, b in zip(cuts, cuts[1:])]
+    tasks = [(wl, inst, a, b, a == b0, threads) for a, b in zip(cuts, cuts[1:])]
     tot = {k: 0 for k in STATS}
     raw = []
-    with ProcessPoolExecutor(int(os.environ.get("GOLDEN_JOBS", jobs))) as ex:
+    with ProcessPoolExecutor(min(jobs, len(tasks))) as ex:
         for st, r, root in ex.map(_window_part, tasks):
             for k in STATS:
                 tot[k] += st[k] + (root[k] if root is not None else 0)
