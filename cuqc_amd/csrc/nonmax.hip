@@ -601,3 +601,81 @@ int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uin
 
 }  // namespace qc
 
+// ---- RemoveNonMax without allocation ----
+// Every buffer of qc_remove_nonmax_noalloc lives in `scratch`, a file-scope
+// device array: the call allocates nothing.  Its device address is looked up
+// once per device (hipGetSymbolAddress) and kept in scratch_dev[]; the
+// array's name itself is not a device address in host code.
+namespace {
+constexpr size_t NM_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NM_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) char scratch[NM_SCRATCH_BYTES];
+
+namespace qc {
+namespace {
+std::mutex scratch_mu;
+char* scratch_dev[NM_MAX_DEVICES] = {};
+
+int scratch_ptr(char** out) {
+    int dev = 0;
+    QC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NM_MAX_DEVICES) {
+        set_error("device %d: at most %d devices are supported", dev, NM_MAX_DEVICES);
+        return QC_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return QC_EDEVICE;
+        }
+        scratch_dev[dev] = (char*)p;
+    }
+    *out = scratch_dev[dev];
+    return QC_OK;
+}
+}  // namespace
+}  // namespace qc
+
+extern "C" int qc_remove_nonmax_noalloc(uint64_t count, uint64_t n_verts, const uint64_t* d_offsets,
+                                        const uint32_t* d_verts, uint32_t n_vertices, uint64_t* d_out_count,
+                                        uint64_t* d_out_offsets, uint32_t* d_out_verts, void* stream) {
+    using namespace qc;
+    if (!d_out_count || !d_out_offsets || (count && (!d_offsets || !d_verts)) || (n_verts && !d_out_verts)) {
+        set_error("NULL argument");
+        return QC_EINVAL;
+    }
+    if (n_vertices == 0 && n_verts) {
+        set_error("n_vertices must bound every vertex id");
+        return QC_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    char* scr = nullptr;
+    int rc = scratch_ptr(&scr);
+    if (rc) return rc;
+    if (count >= 0xFFFFFFFFull || n_verts >= 0xFFFFFFFFull) {
+        set_error("qc_remove_nonmax_noalloc: %llu sets / %llu vertices exceed 2^32 - 1", (unsigned long long)count,
+                  (unsigned long long)n_verts);
+        return QC_EOVERFLOW;
+    }
+    const size_t need = count ? nonmax_scratch_bytes(count, n_verts, n_vertices, s) : 0;
+    if (count && !need) return QC_EDEVICE;
+    if (need > NM_SCRATCH_BYTES) {
+        set_error("qc_remove_nonmax_noalloc: %llu sets of %llu vertices (ids < %u) need %zu bytes of scratch; the "
+                  "limit is %zu",
+                  (unsigned long long)count, (unsigned long long)n_verts, n_vertices, need, NM_SCRATCH_BYTES);
+        return QC_EOVERFLOW;
+    }
+    NmInput in;
+    in.k = count;
+    in.nv = n_verts;
+    in.offsets = d_offsets;
+    in.verts = d_verts;
+    in.nkeys = n_vertices;
+    const NmOutput out{d_out_count, d_out_offsets, d_out_verts, nullptr};
+    return nonmax_run(in, scr, NM_SCRATCH_BYTES, out, s);
+}

@@ -183,6 +183,29 @@ void qc_result_free(qc_result* r);
  * raw sets: device canonicalisation, de-duplication and subset removal. */
 int qc_remove_nonmax(uint64_t count, const uint64_t* offsets, const uint32_t* verts, qc_result** out);
 
+/* RemoveNonMax over raw sets already in device memory, with no device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  Every buffer it needs lives in a 64 MiB device array that the
+ * library holds on each GPU it runs on until the process ends; inputs that
+ * need more (about 64 B per set + 20 B per raw vertex + 12 B per vertex id
+ * below n_vertices) fail with QC_EOVERFLOW before anything is launched.
+ *   d_offsets[count + 1], d_verts[n_verts]: the raw sets (device memory),
+ *     n_verts = d_offsets[count]; every vertex id is below n_vertices.
+ *   d_out_count: receives the number of maximal sets, or UINT64_MAX when a
+ *     vertex id is not below n_vertices.
+ *   d_out_offsets[count + 1], d_out_verts[n_verts]: the maximal sets, each
+ *     ascending, in lexicographic order (as qc_remove_nonmax).
+ * The work is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without waiting: outputs are valid once the stream
+ * has reached this point.  It makes no hipMalloc, hipFree, memset or
+ * synchronizing call.  All calls on one GPU share the scratch array, which
+ * keeps the last call's contents: calls whose work overlaps on one GPU (other
+ * streams, other host threads) race and must be ordered by the caller. */
+int qc_remove_nonmax_noalloc(uint64_t count, uint64_t n_verts, const uint64_t* d_offsets, const uint32_t* d_verts,
+                             uint32_t n_vertices, uint64_t* d_out_count, uint64_t* d_out_offsets,
+                             uint32_t* d_out_verts, void* stream);
+
 /* ---- task-cost model (no reference counterpart) ------------------------
  * A 12 -> 64 -> 64 -> 1 ReLU MLP with bfloat16 weights and fp32 accumulation
  * that predicts log(subtree size) of a task from its qc_result_trace
