@@ -146,3 +146,26 @@ def test_qcconv_cli(tmp_path):
     assert r.returncode == 1 and r.stdout.strip() == "invalid graph file"
     r = subprocess.run([CONV], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "Usage" in r.stdout
+
+
+# ---- against the reference's own scripts ------------------------------------
+
+def _fixtures():
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "convert_fixtures.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("i", range(18))
+def test_edge_to_adj_matches_reference_script(tmp_path, i):
+    """Bytes printed by edgeToAdj.py itself (tests/golden/make_convert_fixtures.py)."""
+    case = _fixtures()["edge_to_adj"][i]
+    assert _conv(tmp_path, qc.convert_edge_to_adj, case["input"], bool(case["duplicate"])) == case["output"]
+
+
+@pytest.mark.parametrize("i", range(6))
+def test_rm_empty_matches_reference_script(tmp_path, i):
+    """Bytes written by rmEmptyVert.py itself (tests/golden/make_convert_fixtures.py)."""
+    case = _fixtures()["rm_empty"][i]
+    assert _conv(tmp_path, qc.convert_rm_empty, case["input"]) == case["output"]
