@@ -11,10 +11,18 @@ tasks/s of the fastest GPU form, the per-form enumeration times, the roofline
 of the expansion kernels and the CPU oracle's throughput on a bounded sample
 of the same workload.
 
-Algorithmic bytes per step (the roofline numerator):
-    A = 12 * sum_t + 12 * sum_t_written + 4 * sum_clique_len + 4 * w_adj
-(12 B per task vertex read or written: id, indeg, exdeg; 4 B per emitted
-clique vertex; 4 B per adjacency entry the reference's list scans visit).
+Algorithmic bytes per step (the roofline numerator), counting only work
+that is done inside the window:
+    A = 12 * sum_t + 12 * (sum_t - root_size) + 4 * sum_clique_len
+(12 B per task vertex read, and written once by the parent: every expanded
+task but the root was written; children past a depth window are built but
+never written and are not counted; 4 B per emitted clique vertex).  The
+reference's adjacency-list scans (4 B per entry, w_adj entries) are reported
+beside it as list_scan_bytes but not counted: here every such scan is a test
+of a bit row of the root's 1-hop / 2-hop matrices, which stay in L2 and LDS,
+and on the dense config-1 window the list volume (about 300 GB per step)
+exceeds what HBM could deliver in the kernel's time by two orders of
+magnitude.
 """
 from __future__ import annotations
 
@@ -31,13 +39,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel (expand_persistent) in the
-# headline form, from one rocprofv3 FETCH_SIZE and one WRITE_SIZE pass, each
-# scaled by the counter's reading for a 256 MiB device copy (FETCH x2.0,
-# WRITE x1.0; profiles/hbm_traffic_coexpr8k_depth1.md).  Measured for the
-# library-default schedule (lds_tile 128 / split_chunk 64 / split_min 512 /
-# lds_wide 512);
-# None where not measured.
-TRAFFIC: dict = {"coexpr8k": 7315228880}
+# headline form, keyed by workload label: one rocprofv3 FETCH_SIZE and one
+# WRITE_SIZE pass, each scaled by the counter's reading for a 256 MiB device
+# copy (FETCH x2.0, WRITE x1.0; profiles/hbm_traffic_*.md).  None where not
+# measured.
+TRAFFIC: dict = {}
 
 
 def _log(msg: str) -> None:
@@ -52,13 +58,44 @@ def set_digest(sets) -> str:
 
 
 def algo_bytes(st: dict) -> int:
-    return 12 * st["sum_t"] + 12 * st["sum_t_written"] + 4 * st["sum_clique_len"] + 4 * st["w_adj"]
+    return 12 * st["sum_t"] + 12 * (st["sum_t"] - st["root_size"]) + 4 * st["sum_clique_len"]
+
+
+def cpu_info() -> dict:
+    """Host CPU model (lscpu / /proc/cpuinfo) and the cores this process may use."""
+    model = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc}
 
 
 def cpu_baseline(w, budget_s: float) -> dict:
-    """Time the oracle (single thread) on a bounded sample of the workload:
-    the last k root branches of its window, k doubled until the run takes
-    about budget_s / 2 seconds or more."""
+    """Time the oracle on a bounded sample of the workload: its last k root
+    branches, with the depth limit cut to at most 2 (a single root branch of
+    the config-1 window at depth 3 is half a minute of one core), k doubled
+    until the run takes about budget_s / 2 seconds or more.  One thread (the
+    reference's host path has none), and the same sample on every host core
+    as "cpu_best" (not the reference: the oracle's threaded search)."""
     import oracle
     from cuqc_amd import workloads
 
@@ -67,41 +104,51 @@ def cpu_baseline(w, budget_s: float) -> dict:
     b1 = w.root_end
     if b1 < 0:
         b1 = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=0, root_end=0)["stats"]["root_branches"]
-    k, best = 8, None
+    lo = w.root_begin if w.root_begin >= 0 else 0
+    depth = 2 if w.max_depth < 0 else min(w.max_depth, 2)
+    k, best = 1, None
     t_spent = 0.0
     while True:
-        b0 = max(w.root_begin, b1 - k)
+        b0 = max(lo, b1 - k)
         t0 = time.perf_counter()
-        r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=w.max_depth)
+        r = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=depth)
         dt = time.perf_counter() - t0
         t_spent += dt
         best = (r["stats"]["tasks"], dt, b0, b1)
-        if dt >= budget_s / 2 or b0 == w.root_begin or t_spent > budget_s:
+        if dt >= budget_s / 2 or b0 == lo or t_spent > budget_s:
             break
         k *= 4 if dt < budget_s / 64 else 2
     tasks, dt, b0, b1 = best
+    info = cpu_info()
+    nthr = max(1, min(int(info["nproc"] or 1), 64))
+    t0 = time.perf_counter()
+    rm = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=b0, root_end=b1, max_depth=depth, threads=nthr)
+    dtm = time.perf_counter() - t0
+    sample = (f"oracle (C++ restatement of main.cu host path), root branches [{b0},{b1}) of {w.name} at "
+              f"depth <= {depth}: {tasks} tasks")
     return {"value": round(tasks / dt, 1), "unit": "tasks/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C++ restatement of main.cu host path), root branches [{b0},{b1}) of workload "
-                      f"{w.name}: {tasks} tasks in {dt:.2f} s"}
+            "sample": f"{sample} in {dt:.2f} s on one thread", **info,
+            "cpu_best": {"value": round(rm["stats"]["tasks"] / dtm, 1), "unit": "tasks/s", "cores": nthr,
+                         "kind": "port", "sample": f"{sample} in {dtm:.2f} s on {nthr} threads (threaded oracle)"}}
 
 
-def cost_model_jobs(w):
-    """Training traces for the cost model: other windows and graphs than the
-    measured one (workload 0 in full, the cheap end of config 1's root
-    branches in full depth, and the first 8 instances of the config-3 batch
-    in their windows)."""
+def cost_model_jobs():
+    """Training traces for the cost model, from configs 1-3 and disjoint from
+    every measured window: workload 0 in full, config 1 root branches
+    3580-3599 (depth <= 2), config 2 root branches 9255-9262 (full depth),
+    and 8 config-3 instances drawn with another seed (44) in the batch's
+    window shape (last 32 root branches, depth <= 2)."""
     from cuqc_amd import workloads
-    jobs = []
-    if w.config != 0:
-        g0 = workloads.graph(workloads.WORKLOADS[0])
-        jobs.append((g0, g0.gamma, g0.min_size, None))
+    g0 = workloads.graph(workloads.WORKLOADS[0])
+    jobs = [(g0, g0.gamma, g0.min_size, None)]
     g1 = workloads.graph(workloads.WORKLOADS[1])
-    jobs.append((g1, g1.gamma, g1.min_size, (3651, 4163, -1)))
-    w3 = workloads.WORKLOADS[3]
-    first = 8 if w.config != 3 else 0
-    for i in range(first):
-        g3 = workloads.graph(w3, i)
-        jobs.append((g3, g3.gamma, g3.min_size, w3.window))
+    jobs.append((g1, g1.gamma, g1.min_size, (3580, 3600, 2)))
+    g2 = workloads.graph(workloads.WORKLOADS[2])
+    jobs.append((g2, g2.gamma, g2.min_size, (9255, 9263, -1)))
+    from cuqc_amd import Graph
+    for i in range(8):
+        g3 = Graph.generate(3, 44, i)
+        jobs.append((g3, g3.gamma, g3.min_size, workloads.WORKLOADS[3].window))
     return jobs
 
 
@@ -129,7 +176,7 @@ class Job:
             enum_ms = res[0].stats["t_enum_ms"]
             kern_ms = res[0].stats["t_kernel_ms"]
         st = {k: sum(r.stats[k] for r in res) for k in ("tasks", "raw_cliques", "sum_t", "sum_t_written",
-                                                        "sum_clique_len", "w_adj")}
+                                                        "sum_clique_len", "w_adj", "root_size", "beyond")}
         h = hashlib.sha256()
         for r in res:
             h.update(set_digest(r.maximal).encode())
@@ -138,23 +185,38 @@ class Job:
 
 
 SCHEDULE_CACHE = None
+_MODEL = {}
 
 
-def setup_forms(job: Job, forms):
-    """Untimed set-up: the autotuned schedule and the trained cost model."""
-    from cuqc_amd import autotune, costmodel
+def trained_model():
+    """The cost model, trained once per run on cost_model_jobs() (untimed)."""
+    if "m" not in _MODEL:
+        from cuqc_amd import costmodel
+        t0 = time.perf_counter()
+        model, rows = costmodel.train_from_traces(cost_model_jobs())
+        _MODEL["m"] = model
+        _MODEL["info"] = {"trace_rows": rows, "seconds": round(time.perf_counter() - t0, 2),
+                          "traces": "workload 0; coexpr8k[3580:3600,depth<=2]; chunglu100k[9255:9263,full depth]; "
+                                    "8 config-3 graphs of seed 44 [-32:,depth<=2]"}
+    return _MODEL["m"], _MODEL["info"]
+
+
+def setup_forms(job: Job, forms, tune: bool = True):
+    """Untimed set-up: the autotuned schedule (the cost form runs on it when
+    `tune`) and the trained cost model."""
+    from cuqc_amd import autotune
     info, kws = {}, {"plain": dict(mode="plain"), "persistent": dict(mode="persistent")}
-    sched = None
-    if "tuned" in forms or "cost" in forms:
+    sched_kw = {}
+    if "tuned" in forms or ("cost" in forms and tune):
         t0 = time.perf_counter()
         sched = autotune.tune(job.gs, job.gammas, job.min_sizes, job.windows, reps=1, log=_log, cache=SCHEDULE_CACHE)
         info["autotune"] = dict(sched.kwargs(), trials=len(sched.trials), seconds=round(time.perf_counter() - t0, 2))
-        kws["tuned"] = dict(mode="persistent", **sched.kwargs())
+        sched_kw = sched.kwargs()
+        kws["tuned"] = dict(mode="persistent", **sched_kw)
     if "cost" in forms:
-        t0 = time.perf_counter()
-        model, rows = costmodel.train_from_traces(cost_model_jobs(job.w))
-        info["cost_model"] = {"trace_rows": rows, "seconds": round(time.perf_counter() - t0, 2)}
-        kws["cost"] = dict(mode="persistent", order="cost", cost_model=model, **sched.kwargs())
+        model, minfo = trained_model()
+        info["cost_model"] = minfo
+        kws["cost"] = dict(mode="persistent", order="cost", cost_model=model, **sched_kw)
     return kws, info
 
 
@@ -175,8 +237,8 @@ def run_form(job: Job, form: str, kw: dict, steps: int, warmup: int) -> dict:
                 wall_ms=sum(r["wall_ms"] for r in runs) / steps)
 
 
-def measure(job: Job, forms, steps: int, warmup: int, plain_steps: int):
-    kws, setup = setup_forms(job, forms)
+def measure(job: Job, forms, steps: int, warmup: int, plain_steps: int, tune: bool = True):
+    kws, setup = setup_forms(job, forms, tune)
     out, digest = {}, None
     for f in forms:
         # the plain form is the comparison point: no warmup
@@ -207,6 +269,9 @@ def main() -> int:
     ap.add_argument("--batch-workload", type=int, default=3,
                     help="also measure this batch workload (one launch vs one plain run per instance); -1: skip")
     ap.add_argument("--batch-forms", default="plain,persistent,cost")
+    ap.add_argument("--extra-workloads", default="2,4,5",
+                    help="also measure these workloads (plain, persistent and cost-ordered persistent forms)")
+    ap.add_argument("--extra-forms", default="plain,persistent,cost")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule-cache", default=None,
@@ -259,16 +324,18 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": w.name, "generator_config": w.config, "seed": w.seed, "instances": w.instances,
-                   "gamma": g.gamma, "min_size": g.min_size, "n": g.n, "window": list(w.window), "form": best},
+        "config": {"workload": w.label, "generator_config": w.config, "seed": w.seed, "instances": w.instances,
+                   "gamma": g.gamma, "min_size": g.min_size, "n": g.n, "form": best},
         "forms": table,
         "tasks": tasks,
+        "beyond": st["beyond"],
         "raw_cliques": st["raw_cliques"],
         "maximal_cliques": out[best]["maximal"],
         "maximal_sha256": digest,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC.get(w.name),
-                     "algorithmic_bytes": A, "kernel_ms": round(out[best]["kern_ms"], 3)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": TRAFFIC.get(w.label),
+                     "algorithmic_bytes": A, "list_scan_bytes": 4 * st["w_adj"],
+                     "kernel_ms": round(out[best]["kern_ms"], 3)},
     }
     if "plain" in out and best != "plain":
         line["speedup_vs_plain"] = round(out["plain"]["enum_ms"] / enum_ms, 3)
@@ -279,12 +346,31 @@ def main() -> int:
         bforms = [f for f in a.batch_forms.split(",") if f]
         bout, btable, bsetup, bdigest = measure(bjob, bforms, a.steps, a.warmup, a.plain_steps)
         bbest = "cost" if "cost" in bout else min(bforms, key=lambda f: bout[f]["enum_ms"])
-        line["batch"] = {"workload": wb.name, "instances": wb.instances, "window": list(wb.window), "form": bbest,
+        line["batch"] = {"workload": wb.label, "instances": wb.instances, "form": bbest,
                          "tasks": bout[bbest]["stats"]["tasks"], "ms_per_step": round(bout[bbest]["enum_ms"], 3),
                          "forms": btable, "maximal_cliques": bout[bbest]["maximal"],
                          "maximal_sha256": bdigest, "setup": bsetup}
         if "plain" in bout and bbest != "plain":
             line["batch"]["speedup_vs_plain"] = round(bout["plain"]["enum_ms"] / bout[bbest]["enum_ms"], 3)
+    extras = {}
+    for k in [int(x) for x in a.extra_workloads.split(",") if x]:
+        if k in (a.workload, a.batch_workload):
+            continue
+        we = workloads.WORKLOADS[k]
+        ejob = Job(qc, we)
+        eforms = [f for f in a.extra_forms.split(",") if f]
+        eout, etable, esetup, edigest = measure(ejob, eforms, a.steps, a.warmup, a.plain_steps, tune=False)
+        ebest = "cost" if "cost" in eout else min(eforms, key=lambda f: eout[f]["enum_ms"])
+        est = eout[ebest]["stats"]
+        extras[we.label] = {"form": ebest, "tasks": est["tasks"], "beyond": est["beyond"],
+                            "raw_cliques": est["raw_cliques"], "maximal_cliques": eout[ebest]["maximal"],
+                            "maximal_sha256": edigest, "ms_per_step": round(eout[ebest]["enum_ms"], 3),
+                            "tasks_per_s": round(est["tasks"] / (eout[ebest]["enum_ms"] / 1e3), 1),
+                            "forms": etable}
+        if "plain" in eout and ebest != "plain":
+            extras[we.label]["speedup_vs_plain"] = round(eout["plain"]["enum_ms"] / eout[ebest]["enum_ms"], 3)
+    if extras:
+        line["workloads"] = extras
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, a.cpu_budget_s)
     print(json.dumps(line), flush=True)

@@ -88,6 +88,7 @@ def load_library() -> C.CDLL:
                                         C.POINTER(C.c_int32)]),
         "qc_graph_csr": (C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
         "qc_graph_checksum": (C.c_uint64, [P]),
+        "qc_graph_planted": (C.c_int, [P, u64p, C.POINTER(u64p), C.POINTER(C.POINTER(C.c_uint32))]),
         "qc_graph_free": (None, [P]),
         "qc_mine": (C.c_int, [P, C.POINTER(_Params), PP]),
         "qc_mine_batch": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(_Params), C.c_int32, PP]),
@@ -214,6 +215,12 @@ class Graph:
 
     def checksum(self) -> int:
         return int(load_library().qc_graph_checksum(self._h))
+
+    def planted(self) -> list[np.ndarray]:
+        """Vertex sets of the quasi-cliques a generated graph was planted with."""
+        cnt, off, v = C.c_uint64(), C.POINTER(C.c_uint64)(), C.POINTER(C.c_uint32)()
+        _check(load_library().qc_graph_planted(self._h, C.byref(cnt), C.byref(off), C.byref(v)))
+        return _sets_from(cnt.value, off, v)
 
     def write_serialized(self, path) -> None:
         _check(load_library().qc_graph_write_serialized(self._h, os.fsencode(path)))

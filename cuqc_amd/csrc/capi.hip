@@ -199,13 +199,13 @@ int guarded(F&& f) {
     }
 }
 
-int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0, double t_init, hipStream_t s) {
+int finish_result(EnumOut& eo, const Instance& I, qc_result* r, double t0, double t_init, hipStream_t s) {
     int rc;
-    r->raw_off = eo.raw_off;
-    r->raw_v = eo.raw_v;
-    r->trace_feat = eo.trace_feat;
-    r->trace_parent = eo.trace_parent;
-    r->trace_subtree = eo.trace_subtree;
+    r->raw_off = std::move(eo.raw_off);
+    r->raw_v = std::move(eo.raw_v);
+    r->trace_feat = std::move(eo.trace_feat);
+    r->trace_parent = std::move(eo.trace_parent);
+    r->trace_subtree = std::move(eo.trace_subtree);
     const double t_enum_end = eo.t_enum_end_ms > 0 ? eo.t_enum_end_ms : now_ms();
     r->st.t_init_ms = t_init - t0;
     r->st.t_enum_ms = t_enum_end - t0;
@@ -225,8 +225,8 @@ int finish_result(const EnumOut& eo, const Instance& I, qc_result* r, double t0,
     r->st.pool_peak = eo.pool_peak;
     r->st.root_size = (uint64_t)I.R;
     if (eo.have_max) {
-        r->max_off = eo.max_off;
-        r->max_v = eo.max_v;
+        r->max_off = std::move(eo.max_off);
+        r->max_v = std::move(eo.max_v);
         r->st.t_nonmax_ms = eo.nonmax_ms;
     } else {
         const double tn = now_ms();
@@ -343,6 +343,14 @@ int qc_graph_csr(const qc_graph* g, int32_t* n, const uint64_t** offsets, const 
     if (n) *n = g->n;
     if (offsets) *offsets = g->off.data();
     if (adj) *adj = g->adj.data();
+    return QC_OK;
+}
+
+int qc_graph_planted(const qc_graph* g, uint64_t* count, const uint64_t** offsets, const uint32_t** verts) {
+    if (!g) return set_error("NULL graph"), QC_EINVAL;
+    if (count) *count = g->planted_off.size() - 1;
+    if (offsets) *offsets = g->planted_off.data();
+    if (verts) *verts = g->planted.data();
     return QC_OK;
 }
 

@@ -53,6 +53,21 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     if (I.R == 0) return QC_OK;
     const uint32_t TL = p.lds_tile == 64 ? 64 : p.lds_tile == 256 ? 256 : 128;
 
+    // slot bound of the root's children inside the search window: branch b
+    // takes root vertex R - 1 - b and writes at most min(R - b, 1 + |N2 n root|)
+    // vertices
+    uint64_t root_bound = 0;
+    {
+        const long long nb = (long long)I.root_branches;
+        long long b0 = p.root_begin >= 0 ? p.root_begin : std::max(0ll, nb + (long long)p.root_begin);
+        long long b1 = p.root_end > 0 ? std::min<long long>(p.root_end, nb) : nb;
+        b0 = std::min(b0, nb);
+        for (long long b = b0; b < b1; b++) {
+            const Vrec& x = I.root[(size_t)(I.R - 1 - b)];
+            root_bound += std::min<uint64_t>((uint64_t)(I.R - b), (uint64_t)I.d2r[x.id] + 1);
+        }
+    }
+
     // ---- capacities ----
     size_t freeb = 0, totalb = 0;
     QC_HIP(hipMemGetInfo(&freeb, &totalb));
@@ -66,7 +81,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     const size_t budget = std::min<size_t>(freeb / 2, (size_t)96 << 30);
     const uint64_t max_slots = budget / 5 / sizeof(Vrec);
     uint64_t stg_slot_cap = p.task_capacity ? p.task_capacity
-                                            : std::min<uint64_t>(max_slots, std::max<uint64_t>(8ull << 20, 2 * I.root_bound));
+                                            : std::min<uint64_t>(max_slots, std::max<uint64_t>(8ull << 20, 2 * root_bound));
     uint64_t pool_cap = p.task_capacity ? p.task_capacity : stg_slot_cap;
     const bool grow = p.task_capacity == 0;
     uint64_t stage_refs = std::min<uint64_t>(stg_slot_cap / 2 + 4096, (uint64_t)128 << 20);
@@ -117,7 +132,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     {
         const int k = I.R <= (int)TL ? 0 : 1;
         QC_HIP(hipMemcpyAsync(st[k].verts.p, I.root.data(), I.root.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
-        TaskRef r{0, (uint32_t)I.R, 0, 0, I.root_bound, 0, 0};
+        TaskRef r{0, (uint32_t)I.R, 0, 0, root_bound, 0, 0};
         st[k].refs.push_back(r);
         st[k].top = I.R;
     }

@@ -1041,21 +1041,22 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
 // ---------------------------------------------------------------------------
 // persistent mode: vertex pool, shared rings, private stacks
 // ---------------------------------------------------------------------------
-constexpr unsigned long long FL_IDX = (1ull << 36) - 1;  // block index field; all ones = empty list
+constexpr unsigned long long FL_IDX = (1ull << 36) - 1;  // block index field (units); all ones = empty list
 
 // Free blocks of the small classes (up to 8 units) first go on a free list
 // of the wave that frees them (heads in wave_fl, only that wave's lane 0
-// touches them: no atomics); up to 64 units per class stay there, the rest
-// and every larger block go on the class's shared lock-free list.  Tiny
-// tasks free and allocate a block each, and one contended list head would
+// touches them: no atomics); up to 4 blocks per class stay there (enough
+// for a task's free and its children's allocations to meet), the rest and
+// every larger block go on the class's shared lock-free list.  Tiny tasks
+// free and allocate a block each, and one contended list head would
 // serialize them.
-constexpr uint32_t FL_LOCAL_UNITS = 64;
+constexpr unsigned long long FL_LOCAL_BLOCKS = 4;
 
 // A block of the record's size class (lane 0): this wave's own list, the
 // class's shared list, else fresh units from the bump pointer.  Returns the
 // Vrec offset, or ~0 when the pool is exhausted (the bump pointer keeps
 // counting: the host reads what the launch needed).
-__device__ unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
+__device__ __attribute__((always_inline)) unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
     uint32_t units = 0;
     const int cl = pool_class(nvrec, units);
     if (units <= 8) {
@@ -1063,9 +1064,9 @@ __device__ unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
         const unsigned long long lh = *lp;
         if ((lh & FL_IDX) != FL_IDX) {
             const unsigned long long idx = lh & FL_IDX;
-            const unsigned long long nxt = *(const unsigned long long*)(a.pool + idx * 64);
+            const unsigned long long nxt = *(const unsigned long long*)(a.pool + idx * POOL_UNIT);
             *lp = (((lh >> 36) - 1ull) << 36) | (nxt & FL_IDX);
-            return idx * 64;
+            return idx * POOL_UNIT;
         }
     }
     unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
@@ -1076,28 +1077,28 @@ __device__ unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
         // takes the block meanwhile, the tag in the head changes and the
         // exchange fails
         const unsigned long long nxt =
-            __hip_atomic_load((unsigned long long*)(a.pool + idx * 64), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_load((unsigned long long*)(a.pool + idx * POOL_UNIT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long nh = (((h >> 36) + 1ull) << 36) | (nxt & FL_IDX);
         if (__hip_atomic_compare_exchange_strong(hp, &h, nh, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
                                                  __HIP_MEMORY_SCOPE_AGENT))
-            return idx * 64;
+            return idx * POOL_UNIT;
     }
-    const unsigned long long n = (unsigned long long)units * 64;
+    const unsigned long long n = (unsigned long long)units * POOL_UNIT;
     const unsigned long long off = atomicAdd(a.pool_top, n);
     return off + n <= a.pool_cap ? off : ~0ull;
 }
 
 // Gives a block back (lane 0, after the wave has read the record: the
 // release of a shared push orders those reads before any reuse).
-__device__ void pool_free(const ExpandArgs& a, unsigned long long off, uint32_t nvrec) {
+__device__ __attribute__((always_inline)) void pool_free(const ExpandArgs& a, unsigned long long off, uint32_t nvrec) {
     uint32_t units = 0;
     const int cl = pool_class(nvrec, units);
-    const unsigned long long idx = off / 64;
+    const unsigned long long idx = off / POOL_UNIT;
     if (units <= 8) {
         unsigned long long* lp = a.wave_fl + (size_t)blockIdx.x * POOL_NCLASS + cl;
         const unsigned long long lh = *lp;
         const unsigned long long cnt = (lh & FL_IDX) == FL_IDX ? 0ull : lh >> 36;
-        if ((cnt + 1) * units <= FL_LOCAL_UNITS) {
+        if (cnt < FL_LOCAL_BLOCKS) {
             *(unsigned long long*)(a.pool + off) = lh & FL_IDX;
             *lp = ((cnt + 1ull) << 36) | idx;
             return;
@@ -1119,7 +1120,7 @@ __device__ void pool_free(const ExpandArgs& a, unsigned long long off, uint32_t 
 // (whole wave; lane j checks the cell of position pos + j).  Returns how
 // many were claimed (0: the ring is full), their first position in *pos.
 // With a trace the ring must not wrap (positions index the trace rows).
-__device__ int ring_claim(Ctx& c, int b, int n, unsigned long long* pos_out) {
+__device__ __attribute__((always_inline)) int ring_claim(Ctx& c, int b, int n, unsigned long long* pos_out) {
     const ExpandArgs& a = *c.a;
     unsigned long long* tp = &a.q_tail[b * QSTRIDE];
     const unsigned long long* seq = a.q_seq + (size_t)b * a.q_cap;
@@ -1174,7 +1175,7 @@ __device__ __forceinline__ void ring_publish(Ctx& c, int b, unsigned long long p
 // One task ref into a ring (shared = true first) or this wave's private
 // stack, counted in `pending` before anyone can take it.  Returns false
 // when both are full (the launch then reports a capacity overflow).
-__device__ bool push_ref(Ctx& c, const TaskRef& r, int bkt, bool shared, const float* feat) {
+__device__ __attribute__((always_inline)) bool push_ref(Ctx& c, const TaskRef& r, int bkt, bool shared, const float* feat) {
     const ExpandArgs& a = *c.a;
     for (int pass = 0; pass < 2; pass++) {
         const bool ring = (pass == 0) == (shared || !a.use_priv);
@@ -1209,6 +1210,28 @@ __device__ bool push_ref(Ctx& c, const TaskRef& r, int bkt, bool shared, const f
         atomicOr(a.error, 1);
     }
     return false;
+}
+
+// Static order: moves the staged children on this wave's stack into the
+// ring, up to 64 per claim, oldest first (lane j copies entry j; the
+// barrier makes lane 0's earlier stack stores visible to the other lanes).
+// Returns false when the ring took none (it is full): the wave then runs
+// its staged tasks itself.
+__device__ __attribute__((always_inline)) bool static_flush(Ctx& c) {
+    const ExpandArgs& a = *c.a;
+    bool moved = false;
+    wsync();
+    while (c.ptop != c.pbot) {
+        unsigned long long pos = 0;
+        const int k = ring_claim(c, 0, (int)min<uint32_t>(c.ptop - c.pbot, (uint32_t)WAVE), &pos);
+        if (k == 0) break;
+        if (c.lane < k)
+            a.q_refs[(pos + c.lane) % a.q_cap] = c.pstack[(c.pbot + (uint32_t)c.lane) % a.priv_cap];
+        ring_publish(c, 0, pos, k);
+        c.pbot += (uint32_t)k;
+        moved = true;
+    }
+    return moved;
 }
 
 // TaskRef as eight dwords, broadcast from lane l
@@ -1279,11 +1302,12 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     r.b0 = 0;
     r.b1 = 0;
     if (q) {
-        // dynamic order: children stay on this wave's stack (depth first)
-        // unless the model predicts the largest subtrees, which go to the
-        // shared rings at once; static order: every task goes through the
-        // rings, whose positions are dealt to the waves in turn
-        const bool shared = !a.use_priv || a.static_sched || (a.n_buckets > 1 && bucket == a.n_buckets - 1);
+        // dynamic order: children stay on this wave's stack (depth first),
+        // with their predicted bucket when a cost model orders the work;
+        // static order: the stack only stages them, and the wave moves them
+        // to the ring (whose positions are dealt to the waves in turn) in
+        // one claim when the task ends
+        const bool shared = !a.use_priv;
         if (!push_ref(c, r, bucket, shared, feat)) return;
     } else {
         // slot bound of this child's own children (see TaskRef)
@@ -1976,9 +2000,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     // the single ring
     unsigned long long spos = blockIdx.x;
 
-    // positions claimed from a ring and not yet run: lane q holds the ref
-    // of position qh + q
-    TaskRef mine;
+    // positions claimed from a ring and not yet run: their refs wait in LDS
+    // (registers are the scarce resource of this kernel)
+    __shared__ TaskRef s_claim[4];
     int nq = 0, qi = 0;
     unsigned long long qbase = 0, qh = 0;
     for (;;) {
@@ -1987,9 +2011,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
         TaskRef ref;
         long long gslot;
         if (qi < nq) {
-            ref = bcast_ref(mine, qi);
+            ref = s_claim[qi];
             gslot = (long long)(qbase + (qh + (unsigned long long)qi) % a.q_cap);
             qi++;
+        } else if (a.static_sched && c.ptop != c.pbot && static_flush(c)) {
+            continue;
         } else if (c.ptop != c.pbot) {
             // this wave's own stack (depth first)
             idle = 0;
@@ -2010,6 +2036,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                 if (shared < G / 4) {
                     const int want = (int)min<unsigned long long>(
                         min<unsigned long long>((unsigned long long)(c.ptop - c.pbot - 1), 16ull), G / 4 - shared);
+                    if (a.n_buckets > 1 && c.lane == 0) {
+                        // cost order: among the 16 oldest records, hand over
+                        // the ones the model predicts largest first
+                        const uint32_t span = min<uint32_t>(c.ptop - c.pbot - 1, 16u);
+                        for (int d = 0; d < want; d++) {
+                            const uint32_t at = (c.pbot + (uint32_t)d) % a.priv_cap;
+                            uint32_t best = at;
+                            int bb = c.pbkt[at];
+                            for (uint32_t e = (uint32_t)d + 1; e < span; e++) {
+                                const uint32_t x = (c.pbot + e) % a.priv_cap;
+                                if (c.pbkt[x] > bb) {
+                                    bb = c.pbkt[x];
+                                    best = x;
+                                }
+                            }
+                            if (best != at) {
+                                const TaskRef tr = c.pstack[at];
+                                c.pstack[at] = c.pstack[best];
+                                c.pstack[best] = tr;
+                                c.pbkt[best] = c.pbkt[at];
+                                c.pbkt[at] = (unsigned char)bb;
+                            }
+                        }
+                    }
                     int bk = 0;
                     if (c.lane == 0) bk = c.pbkt[c.pbot % a.priv_cap];
                     bk = rdlane(bk, 0);
@@ -2111,7 +2161,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             // and gives the cell back for the next lap
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             const unsigned long long base = (unsigned long long)bsel * a.q_cap;
-            if (c.lane < nk) mine = a.q_refs[base + (h + c.lane) % a.q_cap];
+            if (c.lane < nk) s_claim[c.lane] = a.q_refs[base + (h + c.lane) % a.q_cap];
             wsync();
             if (c.lane < nk)
                 __hip_atomic_store(&a.q_seq[base + (h + c.lane) % a.q_cap], h + (unsigned long long)c.lane + a.q_cap,

@@ -171,8 +171,9 @@ void chung_lu_edges(Rng& r, int n, double mean_deg, double beta, Edges& e) {
     }
 }
 
-// Plant `sizes.size()` quasi-cliques on disjoint random vertex sets.
-void plant(Rng& r, int n, const std::vector<int>& sizes, double g, Edges& e) {
+// Plant `sizes.size()` quasi-cliques on disjoint random vertex sets; their
+// member lists (ascending) are appended to `sets`.
+void plant(Rng& r, int n, const std::vector<int>& sizes, double g, Edges& e, std::vector<std::vector<uint32_t>>& sets) {
     int total = 0;
     for (int k : sizes) total += k;
     if (total > n) return;
@@ -190,6 +191,7 @@ void plant(Rng& r, int n, const std::vector<int>& sizes, double g, Edges& e) {
         std::vector<uint32_t> mem(perm.begin() + at, perm.begin() + at + k);
         at += k;
         std::sort(mem.begin(), mem.end());
+        sets.push_back(mem);
         // remove existing internal edges, then keep each pair w.p. 0.95
         std::vector<std::vector<char>> in(k, std::vector<char>(k, 0));
         for (int a = 0; a < k; a++)
@@ -277,11 +279,12 @@ void cosine_edges(Rng& r, int n, int rank, int nmod, double sigma, double noise,
 int generate(int config, uint64_t seed, int instance, qc_graph* g, double* gamma, int* min_size) {
     Edges e;
     int n = 0;
+    std::vector<std::vector<uint32_t>> planted;
     if (config == 0) {
         Rng r(seed);
         n = 200;
         er_edges(r, n, 0.08, e);
-        plant(r, n, {20, 22, 25}, 0.9, e);
+        plant(r, n, {20, 22, 25}, 0.9, e, planted);
         *gamma = 0.9;
         *min_size = 10;
     } else if (config == 1) {
@@ -296,7 +299,7 @@ int generate(int config, uint64_t seed, int instance, qc_graph* g, double* gamma
         chung_lu_edges(r, n, 20.0, 2.1, e);
         std::vector<int> sizes(50);
         for (auto& k : sizes) k = 20 + (int)r.below(41);
-        plant(r, n, sizes, 0.85, e);
+        plant(r, n, sizes, 0.85, e, planted);
         *gamma = 0.85;
         *min_size = 20;
     } else if (config == 3) {
@@ -313,7 +316,7 @@ int generate(int config, uint64_t seed, int instance, qc_graph* g, double* gamma
         int np = 1 + (int)r.below(10);
         std::vector<int> sizes(np);
         for (auto& k : sizes) k = 10 + (int)r.below(31);
-        plant(r, n, sizes, gm, e);
+        plant(r, n, sizes, gm, e, planted);
         *gamma = gm;
         *min_size = ms;
     } else if (config == 4) {
@@ -322,14 +325,23 @@ int generate(int config, uint64_t seed, int instance, qc_graph* g, double* gamma
         chung_lu_edges(r, n, 30.0, 2.2, e);
         std::vector<int> sizes(200);
         for (auto& k : sizes) k = 30 + (int)r.below(51);
-        plant(r, n, sizes, 0.9, e);
+        plant(r, n, sizes, 0.9, e, planted);
         *gamma = 0.9;
         *min_size = 40;
     } else {
         set_error("unknown generator config %d", config);
         return QC_EINVAL;
     }
-    return csr_from_edges(n, e, g);
+    const int rc = csr_from_edges(n, e, g);
+    if (rc == QC_OK) {
+        g->planted_off.assign(1, 0);
+        g->planted.clear();
+        for (auto& m : planted) {
+            g->planted.insert(g->planted.end(), m.begin(), m.end());
+            g->planted_off.push_back(g->planted.size());
+        }
+    }
+    return rc;
 }
 
 }  // namespace qc

@@ -28,7 +28,7 @@ __global__ void expand_persistent(const ExpandArgs*);
 // top ring (published), every other cell free for its first lap, the free
 // lists empty, counters reset.  ctr = heads[nb], tails[nb], pool top,
 // pending, overflow, QSTRIDE words apart.  Root records sit at the start of
-// the pool; allocated blocks start at the next 64-Vrec boundary.
+// the pool; allocated blocks start at the next unit boundary.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
                               uint64_t root_slots, TaskRef* q_refs, unsigned long long* q_seq, uint64_t q_cap, int nb,
                               Vrec* pool, unsigned long long* ctr, unsigned long long* fl_head,
@@ -58,7 +58,7 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
             ctr[b * QSTRIDE] = 0;                                  // head
             ctr[(nb + b) * QSTRIDE] = b == nb - 1 ? nroots : 0;    // tail
         }
-        ctr[2 * nb * QSTRIDE] = (root_slots + 63) / 64 * 64;  // pool top
+        ctr[2 * nb * QSTRIDE] = (root_slots + POOL_UNIT - 1) / POOL_UNIT * POOL_UNIT;  // pool top
         ctr[(2 * nb + 1) * QSTRIDE] = nroots;                 // pending
         ctr[(2 * nb + 2) * QSTRIDE] = 0;                      // overflow
         clq_ctr->items = 0;
@@ -249,7 +249,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         while (priv_cap * 2 <= std::min<uint64_t>(c0, 8192)) priv_cap *= 2;
     }
     // the roots' records precede the first pool block
-    pool_cap = std::max<uint64_t>(pool_cap, (root_verts.size() + 63) / 64 * 64 + 64);
+    pool_cap = std::max<uint64_t>(pool_cap, (root_verts.size() + POOL_UNIT - 1) / POOL_UNIT * POOL_UNIT + 64);
 
     const bool trace = p.trace != 0;
     DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_tbey, d_rfeat, d_cmw, d_cmn, d_phase, d_tl;
@@ -518,10 +518,33 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     std::vector<uint32_t> hv(cc.slots);
     if (cc.items) QC_HIP(hipMemcpy(hr.data(), ws.clq_refs.p, cc.items * sizeof(CliqueRef), hipMemcpyDeviceToHost));
     if (cc.slots) QC_HIP(hipMemcpy(hv.data(), ws.clq_verts.p, cc.slots * 4, hipMemcpyDeviceToHost));
-    for (auto& r : hr) {
-        EnumOut& o = outs[r.inst];
-        o.raw_v.insert(o.raw_v.end(), hv.begin() + r.off, hv.begin() + r.off + r.len);
-        o.raw_off.push_back(o.raw_v.size());
+    // sets in the order of their vertex slots: for one instance they tile the
+    // vertex stream, which then becomes the result as it is
+    std::sort(hr.begin(), hr.end(), [](const CliqueRef& x, const CliqueRef& y) { return x.off < y.off; });
+    bool tiled = count == 1;
+    for (size_t i = 0; tiled && i < hr.size(); i++)
+        tiled = hr[i].off == (i ? hr[i - 1].off + hr[i - 1].len : 0) && hr[i].inst == 0;
+    if (tiled && (hr.empty() || hr.back().off + hr.back().len == cc.slots)) {
+        EnumOut& o = outs[0];
+        o.raw_off.resize(hr.size() + 1);
+        o.raw_off[0] = 0;
+        for (size_t i = 0; i < hr.size(); i++) o.raw_off[i + 1] = hr[i].off + hr[i].len;
+        o.raw_v = std::move(hv);
+    } else {
+        std::vector<uint64_t> nv(count, 0), ns(count, 0);
+        for (const CliqueRef& r : hr) {
+            nv[r.inst] += r.len;
+            ns[r.inst]++;
+        }
+        for (int i = 0; i < count; i++) {
+            outs[i].raw_v.reserve(nv[i]);
+            outs[i].raw_off.reserve(ns[i] + 1);
+        }
+        for (const CliqueRef& r : hr) {
+            EnumOut& o = outs[r.inst];
+            o.raw_v.insert(o.raw_v.end(), hv.begin() + r.off, hv.begin() + r.off + r.len);
+            o.raw_off.push_back(o.raw_v.size());
+        }
     }
     for (int i = 0; i < count; i++) {
         outs[i].ds = first_stats[i];

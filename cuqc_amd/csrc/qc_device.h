@@ -15,12 +15,13 @@
 constexpr int QSTRIDE = 32;
 
 // Vertex pool of the persistent form: records live in blocks of whole
-// 64-Vrec units, in size classes of 1..8 units and then four steps per
-// octave (10, 12, 14, 16, 20, 24, ...); a freed block goes on its class's
-// lock-free free list and is reused by the next record of that class.
-constexpr int POOL_NCLASS = 104;
+// units of POOL_UNIT Vrecs (256 B), in size classes of 1..8 units and then
+// four steps per octave (10, 12, 14, 16, 20, 24, ...); a freed block goes on
+// a free list of its class and is reused by the next record of that class.
+constexpr uint32_t POOL_UNIT = 16;
+constexpr int POOL_NCLASS = 112;
 __host__ __device__ inline int pool_class(uint32_t nvrec, uint32_t& units) {
-    uint32_t u = (nvrec + 63u) / 64u;
+    uint32_t u = (nvrec + POOL_UNIT - 1) / POOL_UNIT;
     if (u == 0) u = 1;
     if (u <= 8) {
         units = u;

@@ -15,6 +15,9 @@ struct qc_graph {
     int32_t n = 0;
     std::vector<uint64_t> off;   // n + 1
     std::vector<uint32_t> adj;   // off[n]
+    // generated graphs: the vertex sets of the planted quasi-cliques
+    std::vector<uint64_t> planted_off{0};
+    std::vector<uint32_t> planted;
     bool has_n2 = false;
     std::vector<uint64_t> off2;  // n + 1 when has_n2
     std::vector<uint32_t> adj2;

@@ -133,6 +133,11 @@ int qc_graph_from_csr(int32_t n, const uint64_t* offsets, const uint32_t* adj, q
 int qc_graph_generate(int32_t config, uint64_t seed, int32_t instance, qc_graph** out, double* gamma,
                       int32_t* min_size);
 
+/* The quasi-cliques a generated graph was planted with (no reference
+ * counterpart: ground truth for tests), `count` ascending vertex sets;
+ * zero sets for graphs that were not generated. */
+int qc_graph_planted(const qc_graph* g, uint64_t* count, const uint64_t** offsets, const uint32_t** verts);
+
 /* Read-only views; pointers stay valid until qc_graph_free. */
 int qc_graph_csr(const qc_graph* g, int32_t* n, const uint64_t** offsets, const uint32_t** adj);
 uint64_t qc_graph_checksum(const qc_graph* g); /* FNV-1a over offsets and adjacency */

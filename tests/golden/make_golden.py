@@ -1,13 +1,15 @@
 """Regenerate the golden fixtures in this folder with the CPU oracle.
 
-    python tests/golden/make_golden.py [--windows]
+    python tests/golden/make_golden.py [--windows 1,2] [--leak-windows 1,2]
 
 workload0.json      generator checksum, oracle counters and maximal sets of
                     workload 0 (full search)
 small_vectors.json  seeded small graphs (CSR inline) with task counts and
                     maximal sets
-map_leak.json       cases where the reference's stale map entries change the
-                    raw output
+map_leak.json       faithful against clean map mode (the reference's stale map
+                    entries, main.cu:1694-1697): cases whose raw output
+                    differs, and stale positions per case (--leak-windows
+                    adds workload windows)
 windows.json        (--windows) per-workload window results: counters and the
                     SHA-256 of the maximal-set file, computed by running the
                     oracle over disjoint root-branch ranges in parallel
@@ -65,18 +67,35 @@ def small_vectors():
     return out
 
 
-def map_leak():
-    from cuqc_amd import Graph
+def _leak_case(args):
+    """Faithful (stale map entries carried over, main.cu:1694-1697) against
+    clean map mode on one graph and window: do the raw sets or the task
+    count differ, and did any stale position fall outside its record."""
+    name, off, adj, gamma, ms, win = args
+    kw = dict(root_begin=win[0], root_end=win[1], max_depth=win[2]) if win else {}
+    a = oracle.mine(off, adj, gamma, ms, faithful=False, **kw)
+    b = oracle.mine(off, adj, gamma, ms, faithful=True, **kw)
+    differs = oracle.canon(a["raw"]) != oracle.canon(b["raw"]) or a["stats"]["tasks"] != b["stats"]["tasks"]
+    return name, differs, b["stats"]["stale_oob"], b["stats"]["stale_hits"]
+
+
+def map_leak(windows=()):
+    from cuqc_amd import Graph, workloads
     g = Graph.generate(0, 1)
-    cases = [("config0", *g.csr(), g.gamma, g.min_size)]
-    cases += [(n, *b(), gm, ms) for n, b, gm, ms in SMALL_CASES]
-    diffs = []
-    for name, off, adj, gamma, ms in cases:
-        a = oracle.mine(off, adj, gamma, ms, faithful=False)
-        b = oracle.mine(off, adj, gamma, ms, faithful=True)
-        if oracle.canon(a["raw"]) != oracle.canon(b["raw"]) or a["stats"]["tasks"] != b["stats"]["tasks"]:
-            diffs.append(name)
-    return {"differs": diffs}
+    cases = [("config0", *g.csr(), g.gamma, g.min_size, None)]
+    cases += [(n, *b(), gm, ms, None) for n, b, gm, ms in SMALL_CASES]
+    for wl in windows:
+        w = workloads.WORKLOADS[wl]
+        gw = workloads.graph(w)
+        cases.append((w.label, *gw.csr(), gw.gamma, gw.min_size, w.window))
+    out = {"differs": [], "stale_oob": {}, "stale_hits": {}}
+    with ProcessPoolExecutor(int(os.environ.get("GOLDEN_JOBS", 8))) as ex:
+        for name, differs, oob, hits in ex.map(_leak_case, cases):
+            if differs:
+                out["differs"].append(name)
+            out["stale_oob"][name] = oob
+            out["stale_hits"][name] = hits
+    return out
 
 
 def _window_part(args):
@@ -156,13 +175,17 @@ def batch_results(wl: int, jobs: int = 8) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--windows", default="")
+    ap.add_argument("--leak-windows", default="",
+                    help="workloads whose windows map_leak.json also covers (single-threaded: slow)")
     a = ap.parse_args()
     with open(os.path.join(HERE, "workload0.json"), "w") as f:
         json.dump(workload0(), f)
     with open(os.path.join(HERE, "small_vectors.json"), "w") as f:
         json.dump(small_vectors(), f)
-    with open(os.path.join(HERE, "map_leak.json"), "w") as f:
-        json.dump(map_leak(), f, indent=1)
+    if a.leak_windows:
+        res = map_leak([int(x) for x in a.leak_windows.split(",") if x])
+        with open(os.path.join(HERE, "map_leak.json"), "w") as f:
+            json.dump(res, f, indent=1)
     if a.windows:
         path = os.path.join(HERE, "windows.json")
         cur = json.load(open(path)) if os.path.exists(path) else {}
