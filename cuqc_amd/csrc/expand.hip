@@ -243,6 +243,9 @@ struct Ctx {
                                       // window): only then is the child's lvl2adj kept
     // persistent mode: this wave's private stack (entries [pbot, ptop) of a
     // ring of priv_cap refs; uniform counters, lane 0 touches the entries)
+    int shard;                        // this wave's ring shard (blockIdx % q_shards)
+    uint32_t* fl;                     // LDS: free block indices of the small classes, FL_LOCAL_MAX per class
+    uint32_t* fln;                    // LDS: their counts
     TaskRef* pstack;
     unsigned char* pbkt;
     uint32_t pbot, ptop;
@@ -1043,77 +1046,132 @@ __device__ int cost_bucket(Ctx& c, const float* feat /* valid in lane 0 */) {
 // ---------------------------------------------------------------------------
 constexpr unsigned long long FL_IDX = (1ull << 36) - 1;  // block index field (units); all ones = empty list
 
-// Free blocks of the small classes (up to 8 units) first go on a free list
-// of the wave that frees them (heads in wave_fl, only that wave's lane 0
-// touches them: no atomics); up to 4 blocks per class stay there (enough
-// for a task's free and its children's allocations to meet), the rest and
-// every larger block go on the class's shared lock-free list.  Tiny tasks
-// free and allocate a block each, and one contended list head would
-// serialize them.
-constexpr unsigned long long FL_LOCAL_BLOCKS = 4;
+// Free blocks.  Every size class has FL_SHARDS shared lock-free stacks of
+// magazines, and a wave uses shard blockIdx % FL_SHARDS (workgroups go to
+// the XCDs in turn, so a shard's users share an XCD and its L2).  A
+// magazine is a free head block that lists up to FL_LOCAL_MAX - 1 more
+// blocks: one exchange on a shard moves up to fl_mag blocks.  The small
+// classes (up to 8 units) also keep up to 2 fl_mag block indices per wave
+// in LDS (lane 0 only): the free of a task's record and its children's
+// allocations meet there without touching memory.
+// Head block words: [0] next magazine (index), [1] block count, then the
+// other blocks' indices as 32-bit words.
+__device__ __forceinline__ unsigned long long* fl_shard(const ExpandArgs& a, int cl, int sh) {
+    return a.fl_head + ((size_t)cl * FL_SHARDS + sh) * QSTRIDE;
+}
 
-// A block of the record's size class (lane 0): this wave's own list, the
-// class's shared list, else fresh units from the bump pointer.  Returns the
-// Vrec offset, or ~0 when the pool is exhausted (the bump pointer keeps
-// counting: the host reads what the launch needed).
-__device__ __attribute__((always_inline)) unsigned long long pool_alloc(const ExpandArgs& a, uint32_t nvrec) {
-    uint32_t units = 0;
-    const int cl = pool_class(nvrec, units);
-    if (units <= 8) {
-        unsigned long long* lp = a.wave_fl + (size_t)blockIdx.x * POOL_NCLASS + cl;
-        const unsigned long long lh = *lp;
-        if ((lh & FL_IDX) != FL_IDX) {
-            const unsigned long long idx = lh & FL_IDX;
-            const unsigned long long nxt = *(const unsigned long long*)(a.pool + idx * POOL_UNIT);
-            *lp = (((lh >> 36) - 1ull) << 36) | (nxt & FL_IDX);
-            return idx * POOL_UNIT;
-        }
-    }
-    unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
+// Pops a magazine (lane 0): its head block index, or FL_IDX when the shard
+// holds none.  The next link is read from the head block before the
+// exchange; if another wave takes that block meanwhile, the tag in the
+// shard head changes and the exchange fails.
+__device__ __attribute__((always_inline)) unsigned long long mag_pop(const ExpandArgs& a, unsigned long long* hp) {
     unsigned long long h = __hip_atomic_load(hp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     while ((h & FL_IDX) != FL_IDX) {
         const unsigned long long idx = h & FL_IDX;
-        // the next link sits in the free block's first word; if another wave
-        // takes the block meanwhile, the tag in the head changes and the
-        // exchange fails
         const unsigned long long nxt =
             __hip_atomic_load((unsigned long long*)(a.pool + idx * POOL_UNIT), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long nh = (((h >> 36) + 1ull) << 36) | (nxt & FL_IDX);
         if (__hip_atomic_compare_exchange_strong(hp, &h, nh, __ATOMIC_ACQUIRE, __ATOMIC_ACQUIRE,
                                                  __HIP_MEMORY_SCOPE_AGENT))
-            return idx * POOL_UNIT;
+            return idx;
     }
-    const unsigned long long n = (unsigned long long)units * POOL_UNIT;
-    const unsigned long long off = atomicAdd(a.pool_top, n);
-    return off + n <= a.pool_cap ? off : ~0ull;
+    return FL_IDX;
 }
 
-// Gives a block back (lane 0, after the wave has read the record: the
-// release of a shared push orders those reads before any reuse).
-__device__ __attribute__((always_inline)) void pool_free(const ExpandArgs& a, unsigned long long off, uint32_t nvrec) {
-    uint32_t units = 0;
-    const int cl = pool_class(nvrec, units);
-    const unsigned long long idx = off / POOL_UNIT;
-    if (units <= 8) {
-        unsigned long long* lp = a.wave_fl + (size_t)blockIdx.x * POOL_NCLASS + cl;
-        const unsigned long long lh = *lp;
-        const unsigned long long cnt = (lh & FL_IDX) == FL_IDX ? 0ull : lh >> 36;
-        if (cnt < FL_LOCAL_BLOCKS) {
-            *(unsigned long long*)(a.pool + off) = lh & FL_IDX;
-            *lp = ((cnt + 1ull) << 36) | idx;
-            return;
-        }
-    }
-    unsigned long long* hp = a.fl_head + (size_t)cl * QSTRIDE;
+// Pushes the magazine headed by block idx (lane 0; its count and member
+// words are written: the release orders them, and the wave's reads of the
+// blocks' old records, before any reuse).
+__device__ __attribute__((always_inline)) void mag_push(const ExpandArgs& a, unsigned long long* hp, unsigned long long idx) {
     unsigned long long h = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-        __hip_atomic_store((unsigned long long*)(a.pool + off), h & FL_IDX, __ATOMIC_RELAXED,
+        __hip_atomic_store((unsigned long long*)(a.pool + idx * POOL_UNIT), h & FL_IDX, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long nh = (((h >> 36) + 1ull) << 36) | idx;
         if (__hip_atomic_compare_exchange_strong(hp, &h, nh, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT))
             return;
     }
+}
+
+// The shared part of pool_alloc (out of line: it is the rare path, and
+// inlined twice it costs the kernel registers).
+__device__ __attribute__((noinline)) unsigned long long pool_alloc_shared(const ExpandArgs& a, uint32_t* fl, uint32_t* fln,
+                                                                          int cl, uint32_t units) {
+    const int sh = (int)(blockIdx.x % FL_SHARDS);
+    unsigned long long m = mag_pop(a, fl_shard(a, cl, sh));
+    if (m == FL_IDX) {
+        const unsigned long long n = (unsigned long long)units * POOL_UNIT;
+        const unsigned long long off = atomicAdd(a.pool_top, n);
+        if (off + n <= a.pool_cap) return off;
+        for (int k = 1; k < FL_SHARDS && m == FL_IDX; k++) m = mag_pop(a, fl_shard(a, cl, (sh + k) % FL_SHARDS));
+        if (m == FL_IDX) return ~0ull;
+        atomicAdd(a.pool_top, 0ull - n);  // served from a free list after all
+    }
+    if (cl < FL_SMALL) {
+        // the other blocks of the magazine go on the (empty) LDS stack
+        const unsigned long long* w = (const unsigned long long*)(a.pool + m * POOL_UNIT);
+        const uint32_t cnt = (uint32_t)min<unsigned long long>(max<unsigned long long>(w[1], 1ull), FL_LOCAL_MAX);
+        const uint32_t* mem = (const uint32_t*)(w + 2);
+        for (uint32_t j = 0; j + 1 < cnt; j++) fl[cl * FL_LOCAL_MAX + j] = mem[j];
+        fln[cl] = cnt - 1;
+    }
+    return m * POOL_UNIT;
+}
+
+// A block of the record's size class (lane 0): this wave's LDS stack, a
+// magazine of the wave's shard, fresh units from the bump pointer, and once
+// the bump region is used up, a magazine of any other shard.  Returns the
+// Vrec offset, or ~0 when the pool is exhausted (the bump pointer then keeps
+// the failed request: the host reads what the launch needed).
+__device__ __attribute__((always_inline)) unsigned long long pool_alloc(Ctx& c, uint32_t nvrec) {
+    const ExpandArgs& a = *c.a;
+    uint32_t units = 0;
+    const int cl = pool_class(nvrec, units);
+    const bool small = cl < FL_SMALL;
+    if (small) {
+        const uint32_t n = c.fln[cl];
+        if (n > 0) {
+            c.fln[cl] = n - 1;
+            return (unsigned long long)c.fl[cl * FL_LOCAL_MAX + n - 1] * POOL_UNIT;
+        }
+    }
+    return pool_alloc_shared(a, c.fl, c.fln, cl, units);
+}
+
+// The shared part of pool_free (out of line, like pool_alloc_shared).
+__device__ __attribute__((noinline)) void pool_free_shared(const ExpandArgs& a, uint32_t* fl, uint32_t* fln, int cl,
+                                                           unsigned long long idx) {
+    unsigned long long* w = (unsigned long long*)(a.pool + idx * POOL_UNIT);
+    if (cl < FL_SMALL) {
+        const uint32_t n = fln[cl];
+        const uint32_t k = a.fl_mag - 1;
+        uint32_t* mem = (uint32_t*)(w + 2);
+        for (uint32_t j = 0; j < k; j++) mem[j] = fl[cl * FL_LOCAL_MAX + n - k + j];
+        fln[cl] = n - k;
+        w[1] = k + 1;
+    } else {
+        w[1] = 1;
+    }
+    mag_push(a, fl_shard(a, cl, (int)(blockIdx.x % FL_SHARDS)), idx);
+}
+
+// Gives a block back (lane 0, after the wave has read the record).  A full
+// LDS stack hands the block, with the stack's fl_mag - 1 newest entries
+// listed in it, to the shard as one magazine.
+__device__ __attribute__((always_inline)) void pool_free(Ctx& c, unsigned long long off, uint32_t nvrec) {
+    const ExpandArgs& a = *c.a;
+    uint32_t units = 0;
+    const int cl = pool_class(nvrec, units);
+    const unsigned long long idx = off / POOL_UNIT;
+    if (cl < FL_SMALL) {
+        const uint32_t n = c.fln[cl];
+        if (n < 2 * a.fl_mag) {
+            c.fl[cl * FL_LOCAL_MAX + n] = (uint32_t)idx;
+            c.fln[cl] = n + 1;
+            return;
+        }
+    }
+    pool_free_shared(a, c.fl, c.fln, cl, idx);
 }
 
 // Claims up to n (<= 64) consecutive free cells of bucket b for producers
@@ -1172,17 +1230,19 @@ __device__ __forceinline__ void ring_publish(Ctx& c, int b, unsigned long long p
     }
 }
 
-// One task ref into a ring (shared = true first) or this wave's private
-// stack, counted in `pending` before anyone can take it.  Returns false
-// when both are full (the launch then reports a capacity overflow).
+// One task ref into a ring (shared = true first; counted in `pending` before
+// anyone can take it) or this wave's private stack (covered by the wave's
+// own count while it is busy).  Returns false when both are full (the launch
+// then reports a capacity overflow).
 __device__ __attribute__((always_inline)) bool push_ref(Ctx& c, const TaskRef& r, int bkt, bool shared, const float* feat) {
     const ExpandArgs& a = *c.a;
+    const int rg = bkt * a.q_shards + c.shard;
     for (int pass = 0; pass < 2; pass++) {
         const bool ring = (pass == 0) == (shared || !a.use_priv);
         if (ring) {
             unsigned long long pos = 0;
-            if (ring_claim(c, bkt, 1, &pos) == 1) {
-                const unsigned long long gi = (unsigned long long)bkt * a.q_cap + pos % a.q_cap;
+            if (ring_claim(c, rg, 1, &pos) == 1) {
+                const unsigned long long gi = (unsigned long long)rg * a.q_cap + pos % a.q_cap;
                 if (c.lane == 0) {
                     __hip_atomic_fetch_add((unsigned long long*)a.pending, 1ull, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
@@ -1191,13 +1251,11 @@ __device__ __attribute__((always_inline)) bool push_ref(Ctx& c, const TaskRef& r
                     if (a.trace_feat && feat)
                         for (int k = 0; k < 12; k++) a.trace_feat[gi * 12 + k] = feat[k];
                 }
-                ring_publish(c, bkt, pos, 1);
+                ring_publish(c, rg, pos, 1);
                 return true;
             }
         } else if (a.use_priv && c.ptop - c.pbot < a.priv_cap) {
             if (c.lane == 0) {
-                __hip_atomic_fetch_add((unsigned long long*)a.pending, 1ull, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
                 c.pstack[c.ptop % a.priv_cap] = r;
                 c.pbkt[c.ptop % a.priv_cap] = (unsigned char)bkt;
             }
@@ -1227,6 +1285,9 @@ __device__ __attribute__((always_inline)) bool static_flush(Ctx& c) {
         if (k == 0) break;
         if (c.lane < k)
             a.q_refs[(pos + c.lane) % a.q_cap] = c.pstack[(c.pbot + (uint32_t)c.lane) % a.priv_cap];
+        if (c.lane == 0)
+            __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         ring_publish(c, 0, pos, k);
         c.pbot += (uint32_t)k;
         moved = true;
@@ -1268,9 +1329,10 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     if (q && (a.trace_feat || a.n_buckets > 1)) task_features(c, tc, mem, (int)c.depth + 1, gap, feat);
     if (q && a.n_buckets > 1) bucket = cost_bucket(c, feat);
     unsigned long long ri = 0, si = 0;
+    if (q) ph_mark(c, PH_CHILD);
     if (c.lane == 0) {
         if (q) {
-            si = pool_alloc(a, (uint32_t)tc);
+            si = pool_alloc(c, (uint32_t)tc);
         } else {
             ri = atomicAdd(&a.out_ctr[s].items, 1ull);
             si = atomicAdd(&a.out_ctr[s].slots, (unsigned long long)tc);
@@ -1308,7 +1370,9 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
         // to the ring (whose positions are dealt to the waves in turn) in
         // one claim when the task ends
         const bool shared = !a.use_priv;
-        if (!push_ref(c, r, bucket, shared, feat)) return;
+        const bool pushed = push_ref(c, r, bucket, shared, feat);
+        ph_mark(c, PH_POOL);
+        if (!pushed) return;
     } else {
         // slot bound of this child's own children (see TaskRef)
         unsigned long long bnd = 0;
@@ -1631,7 +1695,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // the record plus one Vrec after it whose id counts the items still
         // to run (the last one frees the block)
         unsigned long long slot = 0;
-        if (c.lane == 0) slot = pool_alloc(a, (uint32_t)ts + 1u);
+        if (c.lane == 0) slot = pool_alloc(c, (uint32_t)ts + 1u);
         slot = rdlane64(slot, 0);
         if (slot == ~0ull) {
             if (c.lane == 0) atomicOr(a.error, 1);
@@ -1685,20 +1749,21 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         // (lanes write the refs, one release fence per chunk); what the ring
         // cannot take stays on this wave's stack
         const int top = a.n_buckets - 1;
+        const int rg = top * a.q_shards + c.shard;
         int done = 0;
         while (done < n_items) {
             unsigned long long pos = 0;
-            const int k = ring_claim(c, top, min(n_items - done, WAVE), &pos);
+            const int k = ring_claim(c, rg, min(n_items - done, WAVE), &pos);
             if (k == 0) break;
             if (c.lane == 0)
                 __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             if (c.lane < k) {
-                const unsigned long long gi = (unsigned long long)top * a.q_cap + (pos + c.lane) % a.q_cap;
+                const unsigned long long gi = (unsigned long long)rg * a.q_cap + (pos + c.lane) % a.q_cap;
                 if (a.trace_parent) a.trace_parent[gi] = -2;
                 a.q_refs[gi] = item_ref(done + c.lane);
             }
-            ring_publish(c, top, pos, k);
+            ring_publish(c, rg, pos, k);
             done += k;
         }
         for (; done < n_items; done++)
@@ -1980,6 +2045,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     c.pbkt = a.priv_bkt + (size_t)blockIdx.x * a.priv_cap;
     c.pbot = 0;
     c.ptop = 0;
+    c.shard = (int)(blockIdx.x % (unsigned)a.q_shards);
+    // this wave's stacks of free blocks of the small classes
+    __shared__ uint32_t s_fl[FL_SMALL * FL_LOCAL_MAX];
+    __shared__ uint32_t s_fln[FL_SMALL];
+    c.fl = s_fl;
+    c.fln = s_fln;
+    if (c.lane < FL_SMALL) s_fln[c.lane] = 0;
+    wsync();
     char* const slab = a.slab + (size_t)blockIdx.x * a.slab_bytes;
     const uint32_t mid = a.mid_tcap;
     const uint32_t wide = a.lds_wide;
@@ -2005,9 +2078,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     __shared__ TaskRef s_claim[4];
     int nq = 0, qi = 0;
     unsigned long long qbase = 0, qh = 0;
+    // `pending` counts the tasks waiting in the rings plus the waves that hold
+    // work (claimed positions or a non-empty stack); `hungry` counts the
+    // waves that hold none.  A wave changes them only when it takes work
+    // from the rings, hands work to them or runs out, not per task.
+    bool busy = false;
     for (;;) {
-        if (rdlane(c.lane == 0 ? __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0, 0) & 4)
-            break;
         TaskRef ref;
         long long gslot;
         if (qi < nq) {
@@ -2020,22 +2096,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             // this wave's own stack (depth first)
             idle = 0;
             if (!a.static_sched && c.ptop - c.pbot > 1) {
-                // the rings run low while others wait: hand over the oldest
-                // (shallowest, so largest) records, up to 16 at once, into
-                // the bucket of the oldest one
-                unsigned long long shared = 0;
-                if (c.lane == 0)
-                    for (int b = 0; b < a.n_buckets; b++) {
-                        const unsigned long long tb =
-                            __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const unsigned long long hb =
-                            __hip_atomic_load(&a.q_head[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        shared += tb > hb ? tb - hb : 0;
-                    }
-                shared = rdlane64(shared, 0);
-                if (shared < G / 4) {
-                    const int want = (int)min<unsigned long long>(
-                        min<unsigned long long>((unsigned long long)(c.ptop - c.pbot - 1), 16ull), G / 4 - shared);
+                // more waves wait for work than the rings hold: hand over
+                // the oldest (shallowest, so largest) records, up to 16 at
+                // once, into the bucket of the oldest one.  The rings hold
+                // pending - (G - hungry) tasks; handing over more than the
+                // waiting waves take turns the search breadth first and
+                // multiplies the live records.
+                long long hg = 0, pd = 0;
+                if (c.lane == 0) {
+                    hg = __hip_atomic_load(a.hungry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    pd = __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                hg = (long long)rdlane64((unsigned long long)hg, 0);
+                pd = (long long)rdlane64((unsigned long long)pd, 0);
+                const long long waiting = pd - ((long long)G - hg);
+                const long long need = hg - (waiting > 0 ? waiting : 0);
+                if (need > 0) {
+                    const int want = (int)min<long long>(min<long long>((long long)(c.ptop - c.pbot - 1), 16ll), need);
                     if (a.n_buckets > 1 && c.lane == 0) {
                         // cost order: among the 16 oldest records, hand over
                         // the ones the model predicts largest first
@@ -2062,15 +2139,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                     }
                     int bk = 0;
                     if (c.lane == 0) bk = c.pbkt[c.pbot % a.priv_cap];
-                    bk = rdlane(bk, 0);
+                    const int rg = rdlane(bk, 0) * a.q_shards + c.shard;
                     unsigned long long pos = 0;
-                    const int k = ring_claim(c, bk, want, &pos);
+                    const int k = ring_claim(c, rg, want, &pos);
                     if (k > 0) {
+                        // lane j copies entry j (the barrier makes lane 0's
+                        // stack stores visible to the other lanes)
+                        wsync();
+                        if (c.lane < k)
+                            a.q_refs[(unsigned long long)rg * a.q_cap + (pos + c.lane) % a.q_cap] =
+                                c.pstack[(c.pbot + (uint32_t)c.lane) % a.priv_cap];
                         if (c.lane == 0)
-                            for (int j = 0; j < k; j++)
-                                a.q_refs[(unsigned long long)bk * a.q_cap + (pos + j) % a.q_cap] =
-                                    c.pstack[(c.pbot + (uint32_t)j) % a.priv_cap];
-                        ring_publish(c, bk, pos, k);
+                            __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ring_publish(c, rg, pos, k);
                         c.pbot += (uint32_t)k;
                     }
                 }
@@ -2085,9 +2167,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             int state = 0;  // 0 nothing published, 1 got positions, 2 done, 3 retry now
             unsigned long long h = 0;
             int nk = 1;     // consecutive positions claimed
-            int bsel = 0;   // their bucket
-            if (c.lane == 0) {
-                if (a.static_sched) {
+            int bsel = 0;   // their ring
+            if (a.static_sched) {
+                if (c.lane == 0) {
                     const unsigned long long sq =
                         __hip_atomic_load(&a.q_seq[spos % a.q_cap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (sq == spos + 1ull) {
@@ -2095,60 +2177,93 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                         h = spos;
                         spos += G;
                     }
-                } else {
-                    // highest non-empty bucket first
-                    for (int b = a.n_buckets - 1; b >= 0 && (state == 0 || state == 3); b--) {
-                        unsigned long long* hp = &a.q_head[b * QSTRIDE];
-                        const unsigned long long hb = __hip_atomic_load(hp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const unsigned long long tb =
-                            __hip_atomic_load(&a.q_tail[b * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (hb >= tb) continue;
-                        // a very deep queue hands out several consecutive
+                }
+            } else {
+                // every ring at once: lane l reads the head and tail of ring l
+                // (bucket l / q_shards, shard l % q_shards)
+                const int NS = a.q_shards;
+                const int NR = a.n_buckets * NS;
+                unsigned long long hl = 0, tl = 0;
+                if (c.lane < NR) {
+                    hl = __hip_atomic_load(&a.q_head[c.lane * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    tl = __hip_atomic_load(&a.q_tail[c.lane * QSTRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                const uint64_t m = ballot(c.lane < NR && tl > hl);
+                if (m) {
+                    // the highest bucket with work; in it this wave's shard,
+                    // else the next shard after it that has work
+                    const int b = (63 - __builtin_clzll(m)) / NS;
+                    const uint64_t full = NS == 64 ? ~0ull : (1ull << NS) - 1;
+                    const uint64_t sm = (m >> (b * NS)) & full;
+                    const uint64_t rot = ((sm >> c.shard) | (sm << ((NS - c.shard) & 63))) & full;
+                    const int rsel = b * NS + (c.shard + __builtin_ctzll(rot)) % NS;
+                    const unsigned long long hb = rdlane64(hl, rsel), tb = rdlane64(tl, rsel);
+                    if (c.lane == 0) {
+                        // a very deep ring hands out several consecutive
                         // positions per claim: one head update per claim limits
                         // how fast tiny tasks can be taken.  Only very deep: the
                         // positions of one claim run one after another, which
                         // lengthens the critical path when they are heavy.  Only
                         // published positions are claimed.
                         const unsigned long long avail = tb - hb;
-                        int k = avail >= 32 * G ? 4 : avail >= 16 * G ? 2 : 1;
-                        const unsigned long long* seq = a.q_seq + (size_t)b * a.q_cap;
+                        const unsigned long long per = G / (unsigned long long)NS + 1;
+                        const int k = avail >= 32 * per ? 4 : avail >= 16 * per ? 2 : 1;
+                        const unsigned long long* seq = a.q_seq + (size_t)rsel * a.q_cap;
                         int pub = 0;
                         while (pub < k && __hip_atomic_load(&seq[(hb + pub) % a.q_cap], __ATOMIC_RELAXED,
                                                             __HIP_MEMORY_SCOPE_AGENT) == hb + pub + 1ull)
                             pub++;
                         if (pub == 0) {
                             state = 3;  // claimed by a producer, about to be published
-                            continue;
-                        }
-                        unsigned long long e = hb;
-                        if (__hip_atomic_compare_exchange_strong(hp, &e, hb + (unsigned long long)pub, __ATOMIC_RELAXED,
-                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                            state = 1;
-                            h = hb;
-                            nk = pub;
-                            bsel = b;
                         } else {
-                            state = 3;
+                            unsigned long long e = hb;
+                            if (__hip_atomic_compare_exchange_strong(&a.q_head[rsel * QSTRIDE], &e,
+                                                                     hb + (unsigned long long)pub, __ATOMIC_RELAXED,
+                                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                                state = 1;
+                                h = hb;
+                                nk = pub;
+                                bsel = rsel;
+                            } else {
+                                state = 3;
+                            }
                         }
                     }
                 }
-                // every ring looked empty: finished once nothing is pending (a
-                // task counts itself out only after pushing its children).
-                // Polls are relaxed: an acquire invalidates the CU's vector L1
-                // under the waves still working.
-                if (state == 0 && __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
-                    state = 2;
             }
+            // every ring looked empty: finished once nothing is pending (no
+            // ring task and no busy wave: a wave counts itself out only when
+            // its stack is empty, after pushing what it made).  Polls are
+            // relaxed: an acquire invalidates the CU's vector L1 under the
+            // waves still working.
+            if (c.lane == 0 && state == 0 &&
+                __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+                state = 2;
             state = rdlane(state, 0);
             if (state == 2) break;
             if (state == 3) continue;
             if (state == 0) {
+                if (busy) {
+                    // out of work: no longer counted in pending, now hungry
+                    busy = false;
+                    if (c.lane == 0) {
+                        __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)-1ll,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(a.hungry, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    continue;  // look once more before sleeping
+                }
+                // a lost task or another wave's fault: leave (error bit 4)
+                if (rdlane(c.lane == 0 ? __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0,
+                           0) & 4)
+                    break;
                 idle = idle < 16 ? idle + 1 : 16;
                 if (++idle_total >= IDLE_LIMIT) {
                     if (c.lane == 0) atomicOr(a.error, 4);
                     break;
                 }
                 for (int k = 0; k < idle; k++) __builtin_amdgcn_s_sleep(32);
+                ph_mark(c, PH_IDLE);
                 continue;
             }
             idle = 0;
@@ -2156,6 +2271,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             h = rdlane64(h, 0);
             nk = rdlane(nk, 0);
             bsel = rdlane(bsel, 0);
+            // the claimed tasks leave the rings; an idle wave becomes busy
+            if (c.lane == 0) {
+                const long long d = (busy ? 0 : 1) - (long long)nk;
+                if (d) __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)d, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                if (!busy) __hip_atomic_fetch_add(a.hungry, -1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            busy = true;
             // one acquire per claim (pairs with the producers' release before
             // the sequence words), then lane q reads the ref of position h + q
             // and gives the cell back for the next lap
@@ -2172,7 +2295,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             qh = h;
             continue;
         }
-        ph_mark(c, PH_IDLE);
+        ph_mark(c, PH_POP);
         const long long tl_start = a.timeline ? wall_clock64() : 0;
         if (a.graphs) {
             c.g = load_graph(&a.graphs[ref.inst]);
@@ -2251,6 +2374,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             r[3] = (unsigned long long)ref.depth | ((unsigned long long)ref.b0 << 16) |
                    ((unsigned long long)ref.b1 << 40);
         }
+        ph_mark(c, PH_CHILD);
         if (c.lane == 0) {
             // the record's block: a split item's prepared record is freed by
             // the last of its items (count in the Vrec after the record);
@@ -2258,14 +2382,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             if (ref.b1 > 0) {
                 const int left = __hip_atomic_fetch_add(&a.pool[ref.off + ref.t].id, -1, __ATOMIC_ACQ_REL,
                                                         __HIP_MEMORY_SCOPE_AGENT);
-                if (left == 1) pool_free(a, ref.off, ref.t + 1u);
+                if (left == 1) pool_free(c, ref.off, ref.t + 1u);
             } else if (ref.depth > 0) {
-                pool_free(a, ref.off, ref.t);
+                pool_free(c, ref.off, ref.t);
             }
-            __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)-1ll, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        ph_mark(c, PH_CHILD);
+        ph_mark(c, PH_POOL);
     }
     ph_mark(c, PH_IDLE);
     ph_flush(c);

@@ -24,43 +24,47 @@ namespace qcdev {
 template <int TL>
 __global__ void expand_persistent(const ExpandArgs*);
 
-// Queue state for a fresh run: the roots at positions [0, nroots) of the
-// top ring (published), every other cell free for its first lap, the free
-// lists empty, counters reset.  ctr = heads[nb], tails[nb], pool top,
-// pending, overflow, QSTRIDE words apart.  Root records sit at the start of
-// the pool; allocated blocks start at the next unit boundary.
+// Queue state for a fresh run: the roots dealt over the top bucket's ns
+// rings (root i at position i / ns of shard i mod ns, published), every
+// other cell free for its first lap, the free lists empty, counters reset.
+// ctr = heads[nr], tails[nr], pool top, pending, overflow, hungry (nr =
+// nb ns rings), QSTRIDE words apart.  Root records sit at the start of the
+// pool; allocated blocks start at the next unit boundary.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
                               uint64_t root_slots, TaskRef* q_refs, unsigned long long* q_seq, uint64_t q_cap, int nb,
-                              Vrec* pool, unsigned long long* ctr, unsigned long long* fl_head,
-                              unsigned long long* wave_fl, uint32_t n_waves, StreamCtr* clq_ctr,
+                              int ns, Vrec* pool, unsigned long long* ctr, unsigned long long* fl_head,
+                              uint32_t n_waves, StreamCtr* clq_ctr,
                               DevStats* stats, DevStats* inst_stats, uint32_t n_inst, int* error,
                               const float* __restrict__ root_feat, float* trace_feat, long long* trace_parent,
                               unsigned int* trace_beyond) {
     const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t nt = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t top = (uint64_t)(nb - 1) * q_cap;
-    for (uint64_t i = tid; i < (uint64_t)nb * q_cap; i += nt) {
-        const uint64_t pos = i % q_cap;
-        q_seq[i] = (i >= top && pos < nroots) ? pos + 1 : pos;
+    const uint64_t nr = (uint64_t)nb * ns;
+    const uint64_t top = (uint64_t)(nb - 1) * ns;  // first ring of the top bucket
+    for (uint64_t i = tid; i < nr * q_cap; i += nt) {
+        const uint64_t r = i / q_cap, pos = i % q_cap;
+        q_seq[i] = (r >= top && pos * ns + (r - top) < nroots) ? pos + 1 : pos;
     }
-    for (uint64_t i = tid; i < nroots; i += nt) q_refs[top + i] = roots[i];
+    for (uint64_t i = tid; i < nroots; i += nt) q_refs[(top + i % ns) * q_cap + i / ns] = roots[i];
     if (trace_feat) {
+        // (one ring: root i at position i)
         for (uint64_t i = tid; i < (uint64_t)nroots * 12; i += nt) trace_feat[i] = root_feat[i];
         for (uint64_t i = tid; i < nroots; i += nt) trace_parent[i] = -1;
         for (uint64_t i = tid; i < q_cap; i += nt) trace_beyond[i] = 0;
     }
     for (uint64_t i = tid; i < root_slots; i += nt) pool[i] = root_verts[i];
-    for (uint64_t i = tid; i < (uint64_t)POOL_NCLASS; i += nt) fl_head[i * QSTRIDE] = (1ull << 36) - 1;
-    for (uint64_t i = tid; i < (uint64_t)n_waves * POOL_NCLASS; i += nt) wave_fl[i] = (1ull << 36) - 1;
+    for (uint64_t i = tid; i < (uint64_t)POOL_NCLASS * FL_SHARDS; i += nt) fl_head[i * QSTRIDE] = (1ull << 36) - 1;
     for (uint64_t i = tid; i < n_inst; i += nt) inst_stats[i] = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (tid == 0) {
-        for (int b = 0; b < nb; b++) {
-            ctr[b * QSTRIDE] = 0;                                  // head
-            ctr[(nb + b) * QSTRIDE] = b == nb - 1 ? nroots : 0;    // tail
+        for (uint64_t r = 0; r < nr; r++) {
+            ctr[r * QSTRIDE] = 0;  // head
+            // tail: the roots dealt to this ring
+            ctr[(nr + r) * QSTRIDE] = r >= top && r - top < nroots ? (nroots - (r - top) + ns - 1) / ns : 0;
         }
-        ctr[2 * nb * QSTRIDE] = (root_slots + POOL_UNIT - 1) / POOL_UNIT * POOL_UNIT;  // pool top
-        ctr[(2 * nb + 1) * QSTRIDE] = nroots;                 // pending
-        ctr[(2 * nb + 2) * QSTRIDE] = 0;                      // overflow
+        ctr[2 * nr * QSTRIDE] = (root_slots + POOL_UNIT - 1) / POOL_UNIT * POOL_UNIT;  // pool top
+        ctr[(2 * nr + 1) * QSTRIDE] = nroots;                 // pending
+        ctr[(2 * nr + 2) * QSTRIDE] = 0;                      // overflow
+        ctr[(2 * nr + 3) * QSTRIDE] = n_waves;                // hungry: every wave starts without work
         clq_ctr->items = 0;
         clq_ctr->slots = 0;
         *stats = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -80,7 +84,9 @@ size_t ws_bytes_host(uint32_t tcap, bool mat, uint32_t mwords) {
     return (ws_bytes(tcap, mat, mwords) + 255) & ~(size_t)255;
 }
 
-constexpr int NCTR = 11;  // heads, tails (4 buckets at most), pool top, pending, overflow
+constexpr int Q_SHARDS = 8;                 // rings per bucket in the dynamic order: one per XCD
+constexpr int NCTR = 2 * 4 * Q_SHARDS + 4;  // heads, tails (4 buckets x Q_SHARDS rings at most), pool top,
+                                            // pending, overflow, hungry
 
 struct RunState {
     unsigned long long ctr[NCTR];
@@ -93,7 +99,7 @@ struct RunState {
 // serialize on the mutex.  Never destroyed: the runtime may be gone at exit.
 struct Workspace {
     std::mutex mu;
-    DBuf pool, q, seq, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args, fl, wfl, priv, pbkt;
+    DBuf pool, q, seq, slab, clq_refs, clq_verts, ctr, clq_ctr, stats, err, args, fl, priv, pbkt;
     uint64_t pool_cap = 0, q_cap = 0, clq_slots = 0, clq_refs_cap = 0, priv_cap = 0;
 
     static Workspace& get(int dev) {
@@ -224,12 +230,18 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     // launch when a run overflows, unless the caller fixed them
     const bool fixed_tasks = p.task_capacity != 0, fixed_clq = p.clique_capacity != 0;
     const uint64_t max_pool = std::min<uint64_t>(budget / 2, (size_t)32 << 30) / sizeof(Vrec);
-    const uint64_t max_q = (1ull << 28) / nb;
+    // the dynamic order spreads each bucket over Q_SHARDS rings (a wave pushes
+    // to its XCD's ring and takes from it first); static order deals the
+    // positions of one ring, and a trace numbers the tasks by position in one
+    const bool trace = p.trace != 0;
+    const int ns = stat || trace ? 1 : Q_SHARDS;
+    const int nr = nb * ns;
+    const uint64_t max_q = (1ull << 28) / nr;
     const uint64_t max_clq = std::min<uint64_t>(budget / 16 / 4, (uint64_t)1 << 31);
     uint64_t pool_cap = fixed_tasks ? p.task_capacity
                                     : std::max<uint64_t>(ws.pool_cap, std::min<uint64_t>(max_pool, 1ull << 26));
     pool_cap = std::max<uint64_t>(pool_cap, root_verts.size() + 1024);
-    uint64_t q_cap = std::max<uint64_t>(ws.q_cap / nb, std::min<uint64_t>(max_q, 1ull << 22));
+    uint64_t q_cap = std::max<uint64_t>(ws.q_cap / nr, std::min<uint64_t>(max_q, (1ull << 22) / ns));
     q_cap = std::max<uint64_t>(q_cap, roots.size() + 1024);
     uint64_t clq_slots = fixed_clq ? p.clique_capacity
                                    : std::max<uint64_t>(ws.clq_slots, std::min<uint64_t>(max_clq, 1ull << 24));
@@ -251,12 +263,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     // the roots' records precede the first pool block
     pool_cap = std::max<uint64_t>(pool_cap, (root_verts.size() + POOL_UNIT - 1) / POOL_UNIT * POOL_UNIT + 64);
 
-    const bool trace = p.trace != 0;
     DBuf d_istats, d_graphs, d_roots, d_root_verts, d_tfeat, d_tpar, d_tbey, d_rfeat, d_cmw, d_cmn, d_phase, d_tl;
     // diagnostics: QCMINE_TIMELINE=<file> writes one CSV line per task taken
     const char* timeline_path = getenv("QCMINE_TIMELINE");
     if (cost && (rc = model_device_params((const qc_model*)p.cost_model, d_cmw, d_cmn, s))) return rc;
-    if ((rc = ensure(ws.ctr, NCTR * QSTRIDE * 8)) || (rc = ensure(ws.fl, (size_t)POOL_NCLASS * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
+    if ((rc = ensure(ws.ctr, NCTR * QSTRIDE * 8)) || (rc = ensure(ws.fl, (size_t)POOL_NCLASS * FL_SHARDS * QSTRIDE * 8)) || (rc = ensure(ws.clq_ctr, sizeof(StreamCtr))) ||
         (rc = ensure(ws.stats, sizeof(DevStats))) || (rc = ensure(ws.err, sizeof(int))) ||
         (rc = ensure(ws.args, sizeof(ExpandArgs))) || (rc = ensure(ws.slab, (size_t)grid * slab_b)) ||
         (rc = d_istats.alloc((size_t)count * sizeof(DevStats))) ||
@@ -294,17 +305,16 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     std::vector<DevStats> first_stats;
     float first_ms = 0;
     for (int attempt = 0;; attempt++) {
-        if ((rc = ensure(ws.pool, pool_cap * sizeof(Vrec))) || (rc = ensure(ws.q, nb * q_cap * sizeof(TaskRef))) ||
-            (rc = ensure(ws.seq, nb * q_cap * 8)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
+        if ((rc = ensure(ws.pool, pool_cap * sizeof(Vrec))) || (rc = ensure(ws.q, nr * q_cap * sizeof(TaskRef))) ||
+            (rc = ensure(ws.seq, nr * q_cap * 8)) || (rc = ensure(ws.clq_refs, clq_refs * sizeof(CliqueRef))) ||
             (rc = ensure(ws.clq_verts, clq_slots * 4)) || (rc = ensure(ws.priv, (size_t)grid * priv_cap * sizeof(TaskRef))) ||
-            (rc = ensure(ws.pbkt, (size_t)grid * priv_cap)) ||
-            (rc = ensure(ws.wfl, (size_t)grid * POOL_NCLASS * 8)))
+            (rc = ensure(ws.pbkt, (size_t)grid * priv_cap)))
             return rc;
         if (trace && ((rc = d_tfeat.alloc(q_cap * 12 * sizeof(float))) || (rc = d_tpar.alloc(q_cap * 8)) ||
                       (rc = d_tbey.alloc(q_cap * 4))))
             return rc;
         ws.pool_cap = std::max(ws.pool_cap, pool_cap);
-        ws.q_cap = std::max<uint64_t>(ws.q_cap, nb * q_cap);
+        ws.q_cap = std::max<uint64_t>(ws.q_cap, nr * q_cap);
         ws.clq_slots = std::max(ws.clq_slots, clq_slots);
         ws.clq_refs_cap = std::max(ws.clq_refs_cap, clq_refs);
         ws.priv_cap = std::max(ws.priv_cap, priv_cap);
@@ -336,7 +346,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.q_refs = ws.q.as<TaskRef>();
         A.q_seq = ws.seq.as<unsigned long long>();
         A.fl_head = ws.fl.as<unsigned long long>();
-        A.wave_fl = ws.wfl.as<unsigned long long>();
+        // magazine size: the waves' LDS stacks hold at most 2 fl_mag blocks
+        // of each small class (1152 fl_mag Vrecs per wave), kept under an
+        // eighth of the pool
+        A.fl_mag = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(FL_LOCAL_MAX / 2, pool_cap / ((uint64_t)8 * grid * 1152)));
         A.priv = ws.priv.as<TaskRef>();
         A.priv_bkt = ws.pbkt.as<unsigned char>();
         A.priv_cap = (uint32_t)priv_cap;
@@ -344,11 +358,13 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.static_sched = stat ? 1 : 0;
         unsigned long long* ctr = ws.ctr.as<unsigned long long>();
         A.q_head = ctr;
-        A.q_tail = ctr + nb * QSTRIDE;
-        A.pool_top = ctr + 2 * nb * QSTRIDE;
-        A.pending = (long long*)(ctr + (2 * nb + 1) * QSTRIDE);
-        A.overflow = ctr + (2 * nb + 2) * QSTRIDE;
+        A.q_tail = ctr + nr * QSTRIDE;
+        A.pool_top = ctr + 2 * nr * QSTRIDE;
+        A.pending = (long long*)(ctr + (2 * nr + 1) * QSTRIDE);
+        A.overflow = ctr + (2 * nr + 2) * QSTRIDE;
+        A.hungry = (long long*)(ctr + (2 * nr + 3) * QSTRIDE);
         A.n_buckets = nb;
+        A.q_shards = ns;
         A.n_roots = (int32_t)roots.size();
         A.cm_w = cost ? d_cmw.as<uint16_t>() : nullptr;
         A.cm_norm = cost ? d_cmn.as<float>() : nullptr;
@@ -360,8 +376,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.trace_beyond = trace ? d_tbey.as<unsigned int>() : nullptr;
         A.phase = phases ? d_phase.as<unsigned long long>() : nullptr;
         if (timeline_path) {
-            if ((rc = d_tl.alloc(nb * q_cap * 32))) return rc;
-            QC_HIP(hipMemsetAsync(d_tl.p, 0, nb * q_cap * 32, s));
+            if ((rc = d_tl.alloc(nr * q_cap * 32))) return rc;
+            QC_HIP(hipMemsetAsync(d_tl.p, 0, nr * q_cap * 32, s));
             A.timeline = d_tl.as<unsigned long long>();
         }
         QC_HIP(hipMemcpyAsync(ws.args.p, &A, sizeof(A), hipMemcpyHostToDevice, s));
@@ -376,8 +392,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         QC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(qcdev::persist_reset, dim3(1024), dim3(256), 0, s, d_roots.as<TaskRef>(), nroots,
                            d_root_verts.as<Vrec>(), (uint64_t)root_verts.size(), ws.q.as<TaskRef>(),
-                           ws.seq.as<unsigned long long>(), q_cap, nb, ws.pool.as<Vrec>(), ctr,
-                           ws.fl.as<unsigned long long>(), ws.wfl.as<unsigned long long>(), grid,
+                           ws.seq.as<unsigned long long>(), q_cap, nb, ns, ws.pool.as<Vrec>(), ctr,
+                           ws.fl.as<unsigned long long>(), grid,
                            ws.clq_ctr.as<StreamCtr>(), ws.stats.as<DevStats>(), d_istats.as<DevStats>(),
                            (uint32_t)count, ws.err.as<int>(), d_rfeat.as<float>(), A.trace_feat, A.trace_parent,
                            A.trace_beyond);
@@ -425,8 +441,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
             if (rs.err) {
                 if (rs.err == 1 && rep == 0) {
                     // grow what ran out (its counter kept counting) and rerun
-                    const uint64_t pool_need = rs.ctr[2 * nb];
-                    const uint64_t pushes_lost = rs.ctr[2 * nb + 2];
+                    const uint64_t pool_need = rs.ctr[2 * nr];
+                    const uint64_t pushes_lost = rs.ctr[2 * nr + 2];
                     bool grown = false;
                     if (pool_need > pool_cap && !fixed_tasks && pool_cap < max_pool) {
                         pool_cap = std::min<uint64_t>(max_pool, std::max<uint64_t>(pool_cap * 4, pool_need * 2));
@@ -479,7 +495,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         break;
     }
     if (timeline_path) {
-        std::vector<unsigned long long> tl((size_t)nb * q_cap * 4);
+        std::vector<unsigned long long> tl((size_t)nr * q_cap * 4);
         QC_HIP(hipMemcpy(tl.data(), d_tl.p, tl.size() * 8, hipMemcpyDeviceToHost));
         int khz = 0;
         (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
@@ -502,7 +518,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         unsigned long long ph[PH_COUNT];
         QC_HIP(hipMemcpy(ph, d_phase.p, sizeof(ph), hipMemcpyDeviceToHost));
         static const char* names[] = {"load", "lookahead", "remove", "add", "prune", "child", "prepare", "idle",
-                                      "lu", "degupd", "classify", "critical", "compact", "hist"};
+                                      "lu", "degupd", "classify", "critical", "compact", "hist", "pool", "pop"};
         double tot = 0;
         for (int k = 0; k < PH_COUNT; k++) tot += (double)ph[k];
         fprintf(stderr, "[qcmine] phases (grid %u waves, %.3g wave-cycles):", grid, tot);
@@ -550,7 +566,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         outs[i].ds = first_stats[i];
         outs[i].kernel_ms = first_ms;
         outs[i].levels = 1;
-        outs[i].pool_peak = first.ctr[2 * nb];
+        outs[i].pool_peak = first.ctr[2 * nr];
     }
     if (getenv("QCMINE_DEBUG") && atoi(getenv("QCMINE_DEBUG")) > 0) {
         // the reference's size report (main.cu:2635-2646, 2893-2907) for the
@@ -567,9 +583,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
             t.sum_clique_len += d.sum_clique_len;
         }
         printf("L: 1 T1: %llu %llu T2: %llu %llu B: 0 %llu C: %llu %llu\n\n", t.tasks, t.sum_t, t.children,
-               t.sum_t_written, (unsigned long long)first.ctr[2 * nb], t.raw_cliques, t.sum_clique_len);
+               t.sum_t_written, (unsigned long long)first.ctr[2 * nr], t.raw_cliques, t.sum_clique_len);
         printf("\nTASKS SIZE: %llu\nBUFFER SIZE: %llu\nCLIQUES SIZE: %llu\nCLIQUES OFFSET SIZE: %llu\n\n",
-               (unsigned long long)pool_cap, (unsigned long long)first.ctr[2 * nb],
+               (unsigned long long)pool_cap, (unsigned long long)first.ctr[2 * nr],
                (unsigned long long)first.clq.slots, (unsigned long long)first.clq.items);
         fflush(stdout);
     }
@@ -619,7 +635,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         // rows = expanded tasks (split items excluded), per instance; a
         // child's slot is always above its parent's, so one backward pass
         // accumulates subtree sizes
-        const uint64_t tail = std::min<uint64_t>(first.ctr[nb], q_cap);
+        const uint64_t tail = std::min<uint64_t>(first.ctr[nr], q_cap);
         std::vector<float> tf(tail * 12);
         std::vector<long long> tp(tail);
         std::vector<TaskRef> qr(tail);

@@ -20,6 +20,9 @@ constexpr int QSTRIDE = 32;
 // a free list of its class and is reused by the next record of that class.
 constexpr uint32_t POOL_UNIT = 16;
 constexpr int POOL_NCLASS = 112;
+constexpr int FL_SHARDS = 16;     // shared free lists per class (a wave uses blockIdx % FL_SHARDS)
+constexpr int FL_SMALL = 8;       // classes with per-wave LDS stacks (1..8 units)
+constexpr int FL_LOCAL_MAX = 8;   // LDS stack entries per small class
 __host__ __device__ inline int pool_class(uint32_t nvrec, uint32_t& units) {
     uint32_t u = (nvrec + POOL_UNIT - 1) / POOL_UNIT;
     if (u == 0) u = 1;
@@ -161,17 +164,19 @@ struct ExpandArgs {
     int32_t lds_masks;               // persistent mode: byte offset in the dynamic LDS of the
                                      // two set masks of tasks above the LDS tile (-1: slab)
 
-    // persistent mode: n_buckets shared ring queues of task refs plus one
-    // private LIFO stack per wave, over a vertex pool with free lists.
-    // Bucket b owns cells [b q_cap, (b+1) q_cap) of q_refs / q_seq and the
-    // position counters q_head[b QSTRIDE], q_tail[b QSTRIDE] (every shared
-    // counter sits on a 256-byte line of its own).  Position p of a bucket
-    // uses cell p mod q_cap, whose sequence word reads p while the cell is
-    // free for p, p + 1 once the ref of p is published and p + q_cap once it
-    // has been taken.  `pending` counts tasks pushed anywhere and not yet
-    // finished (0 = search complete).  With a cost model, a child's bucket
-    // is its predicted log subtree size; waves take the highest non-empty
-    // bucket first.
+    // persistent mode: n_buckets x q_shards shared ring queues of task refs
+    // plus one private LIFO stack per wave, over a vertex pool with free
+    // lists.  Ring r = b q_shards + s (bucket b, shard s) owns cells
+    // [r q_cap, (r+1) q_cap) of q_refs / q_seq and the position counters
+    // q_head[r QSTRIDE], q_tail[r QSTRIDE] (every shared counter sits on a
+    // 256-byte line of its own).  A wave pushes to its own shard and takes
+    // from it first, then from the other shards of the same bucket.
+    // Position p of a ring uses cell p mod q_cap, whose sequence word reads
+    // p while the cell is free for p, p + 1 once the ref of p is published
+    // and p + q_cap once it has been taken.  `pending` counts the tasks
+    // waiting in the rings plus the waves holding work (0 = search
+    // complete).  With a cost model, a child's bucket is its predicted log
+    // subtree size; waves take the highest non-empty bucket first.
     int32_t queue_mode;
     int32_t split_chunk;             // branches per split item (0: no splitting)
     uint32_t split_min_t;            // split tasks above this size
@@ -183,12 +188,13 @@ struct ExpandArgs {
     Vrec* pool;
     unsigned long long* pool_top;    // bump pointer for blocks no free list holds (Vrec units)
     unsigned long long pool_cap;
-    unsigned long long* fl_head;     // free-list heads, POOL_NCLASS lines of QSTRIDE words:
-                                     // (tag << 36) | block index (64-Vrec units), all ones = empty
-    unsigned long long* wave_fl;     // per wave, POOL_NCLASS heads of its own free lists:
-                                     // (count << 36) | block index, index all ones = empty
+    unsigned long long* fl_head;     // shared stacks of free-block magazines, POOL_NCLASS x FL_SHARDS
+                                     // lines of QSTRIDE words: (tag << 36) | head block index
+                                     // (POOL_UNIT-Vrec units), index all ones = empty
+    uint32_t fl_mag;                 // blocks per magazine of the small classes (1..FL_LOCAL_MAX / 2)
     long long* pending;
     unsigned long long* overflow;    // pushes that found the ring and the private stack full
+    long long* hungry;               // waves holding no work (they ask the busy ones to hand some over)
     // private stacks: wave w owns priv[w priv_cap, (w+1) priv_cap) (refs)
     // and priv_bkt[...] (the bucket each ref would take in the rings);
     // children stay there (depth first) unless use_priv is 0, and the
@@ -207,7 +213,7 @@ struct ExpandArgs {
 
     int32_t n_buckets;               // 1 = FIFO
     int32_t n_roots;                 // root tasks of the launch (root split width)
-    int32_t pad2;
+    int32_t q_shards;                // rings per bucket (ring b q_shards + s; wave w uses shard w mod q_shards)
     const uint16_t* cm_w;            // cost model: bf16 weights (5057)
     const float* cm_norm;            // mu[12] sd[12] ymu ysd thresholds[3]
 
@@ -230,12 +236,14 @@ enum PhaseId {
     PH_PRUNE,         // degree / critical pruning and the clique test
     PH_CHILD,         // child sort and write
     PH_PREPARE,       // split preparation
-    PH_IDLE,          // persistent pop loop: waiting for work
+    PH_IDLE,          // persistent form: waiting for work (empty polls and back-off)
     PH_LU,            // LU bounds and member checks
     PH_DEGUPD,        // degree-pruning exdeg updates
     PH_COMPACT,       // degree-pruning classification
     PH_CRIT,          // critical-vertex step
     PH_MOVE,          // degree-pruning compaction of the kept candidates
     PH_HIST,          // candidate indegree tables for the LU bounds
+    PH_POOL,          // persistent form: record blocks taken and given back, task refs pushed
+    PH_POP,           // persistent form: taking the next task (own stack, hand-over, rings)
     PH_COUNT
 };
