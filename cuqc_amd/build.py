@@ -26,7 +26,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["expand.hip", "root.hip", "driver.hip", "nonmax.hip", "persist.hip", "mlp.hip", "capi.hip"]
+HIP_SOURCES = ["expand.hip", "root.hip", "driver.hip", "nonmax.hip", "collect.hip", "persist.hip", "mlp.hip",
+               "capi.hip"]
 CXX_SOURCES = ["graph_io.cpp", "gen.cpp", "convert.cpp"]
 HEADERS = ["qc_host.h", "qc_device.h", "qc_internal.h"]
 

@@ -494,6 +494,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         }
         break;
     }
+    // the ENUMERATION window ends with the expansion done and its counters
+    // read; the reference stops its timer there too, before it copies the
+    // device clique buffer out (main.cu:683-688)
+    const double t_enum_end = now_steady_ms();
+    for (int i = 0; i < count; i++) outs[i].t_enum_end_ms = t_enum_end;
     if (timeline_path) {
         std::vector<unsigned long long> tl((size_t)nr * q_cap * 4);
         QC_HIP(hipMemcpy(tl.data(), d_tl.p, tl.size() * 8, hipMemcpyDeviceToHost));
@@ -530,36 +535,32 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     // per-instance outputs (the last replay's buffers; replays are identical
     // in every counter, the raw sets only in their order)
     StreamCtr cc = first.clq;
-    std::vector<CliqueRef> hr(cc.items);
-    std::vector<uint32_t> hv(cc.slots);
-    if (cc.items) QC_HIP(hipMemcpy(hr.data(), ws.clq_refs.p, cc.items * sizeof(CliqueRef), hipMemcpyDeviceToHost));
-    if (cc.slots) QC_HIP(hipMemcpy(hv.data(), ws.clq_verts.p, cc.slots * 4, hipMemcpyDeviceToHost));
-    // sets in the order of their vertex slots: for one instance they tile the
-    // vertex stream, which then becomes the result as it is
-    std::sort(hr.begin(), hr.end(), [](const CliqueRef& x, const CliqueRef& y) { return x.off < y.off; });
-    bool tiled = count == 1;
-    for (size_t i = 0; tiled && i < hr.size(); i++)
-        tiled = hr[i].off == (i ? hr[i - 1].off + hr[i - 1].len : 0) && hr[i].inst == 0;
-    if (tiled && (hr.empty() || hr.back().off + hr.back().len == cc.slots)) {
-        EnumOut& o = outs[0];
-        o.raw_off.resize(hr.size() + 1);
-        o.raw_off[0] = 0;
-        for (size_t i = 0; i < hr.size(); i++) o.raw_off[i + 1] = hr[i].off + hr[i].len;
-        o.raw_v = std::move(hv);
-    } else {
-        std::vector<uint64_t> nv(count, 0), ns(count, 0);
-        for (const CliqueRef& r : hr) {
-            nv[r.inst] += r.len;
-            ns[r.inst]++;
-        }
-        for (int i = 0; i < count; i++) {
-            outs[i].raw_v.reserve(nv[i]);
-            outs[i].raw_off.reserve(ns[i] + 1);
-        }
-        for (const CliqueRef& r : hr) {
-            EnumOut& o = outs[r.inst];
-            o.raw_v.insert(o.raw_v.end(), hv.begin() + r.off, hv.begin() + r.off + r.len);
-            o.raw_off.push_back(o.raw_v.size());
+    {
+        std::vector<uint64_t> go;
+        std::vector<uint32_t> gv;
+        if ((rc = collect_raw(ws.clq_refs.as<CliqueRef>(), ws.clq_verts.as<uint32_t>(), cc.items, cc.slots, go, gv,
+                              s)))
+            return rc;
+        if (count == 1) {
+            outs[0].raw_off = std::move(go);
+            outs[0].raw_v = std::move(gv);
+        } else {
+            // instance i holds the next first_stats[i].raw_cliques sets
+            uint64_t at = 0;
+            for (int i = 0; i < count; i++) {
+                const uint64_t n = first_stats[i].raw_cliques;
+                if (at + n > cc.items) break;
+                EnumOut& o = outs[i];
+                o.raw_off.resize(n + 1);
+                for (uint64_t j = 0; j <= n; j++) o.raw_off[j] = go[at + j] - go[at];
+                o.raw_v.assign(gv.begin() + go[at], gv.begin() + go[at + n]);
+                at += n;
+            }
+            if (at != cc.items) {
+                set_error("raw set collection: the instances count %llu sets, the stream %llu",
+                          (unsigned long long)at, (unsigned long long)cc.items);
+                return QC_EDEVICE;
+            }
         }
     }
     for (int i = 0; i < count; i++) {
@@ -589,11 +590,9 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
                (unsigned long long)first.clq.slots, (unsigned long long)first.clq.items);
         fflush(stdout);
     }
-    // the ENUMERATION window ends with the raw sets collected; RemoveNonMax
-    // then runs on the raw sets where the kernel left them, all instances in
-    // one pass (instance i's vertices are keyed from base[i])
-    const double t_enum_end = now_steady_ms();
-    for (int i = 0; i < count; i++) outs[i].t_enum_end_ms = t_enum_end;
+    // RemoveNonMax runs on the raw sets where the kernel left them, all
+    // instances in one pass (instance i's vertices are keyed from base[i])
+    const double t_nm0 = now_steady_ms();
     if (cc.items) {
         std::vector<uint32_t> base(count), limit(count);
         uint64_t nk = 0;
@@ -626,7 +625,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         }
         if ((rc = remove_nonmax_sets(in, count, mo, mv, s))) return rc;
     }
-    const double nm_ms = now_steady_ms() - t_enum_end;
+    const double nm_ms = now_steady_ms() - t_nm0;
     for (int i = 0; i < count; i++) {
         outs[i].have_max = true;
         outs[i].nonmax_ms = nm_ms;

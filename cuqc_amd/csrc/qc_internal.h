@@ -204,6 +204,11 @@ int nonmax_run(const NmInput& in, char* scratch, size_t scratch_bytes, const NmO
 // nonmax_run in allocated scratch, results copied to the host per instance
 int remove_nonmax_sets(const NmInput& in, int n_inst, std::vector<std::vector<uint64_t>*> mo,
                        std::vector<std::vector<uint32_t>*> mv, hipStream_t s);
+// the raw sets of a persistent launch's clique stream (k refs, nv vertex
+// slots) as one array, instances in order and each instance's sets in
+// vertex-slot order, on the host
+int collect_raw(const CliqueRef* d_refs, const uint32_t* d_verts, uint64_t k, uint64_t nv, std::vector<uint64_t>& off,
+                std::vector<uint32_t>& v, hipStream_t s);
 // host raw sets (uploaded once)
 int remove_nonmax_device(const std::vector<uint64_t>& off, const std::vector<uint32_t>& v, std::vector<uint64_t>& mo,
                          std::vector<uint32_t>& mv, hipStream_t s);
