@@ -12,10 +12,11 @@ when none is present.  There is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 import os
+from collections.abc import Sequence
 from dataclasses import dataclass
 from pathlib import Path
-from typing import Sequence
 
 import numpy as np
 
@@ -143,6 +144,69 @@ def _sets_from(count: int, off, verts) -> list[np.ndarray]:
     return [v[o[i]:o[i + 1]] for i in range(count)]
 
 
+class SetList(Sequence):
+    """Vertex sets held as one offsets array (count + 1 entries) and one
+    vertex array; indexing and iteration give each set as an array view, so
+    millions of raw sets cost two arrays, not a list of millions."""
+
+    __slots__ = ("offsets", "verts")
+
+    def __init__(self, offsets: np.ndarray, verts: np.ndarray):
+        self.offsets = offsets
+        self.verts = verts
+
+    @classmethod
+    def from_view(cls, count: int, off, verts) -> "SetList":
+        if count == 0:
+            return cls(np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+        o = np.ctypeslib.as_array(off, shape=(count + 1,)).copy()
+        v = np.ctypeslib.as_array(verts, shape=(int(o[-1]),)).copy() if o[-1] else np.zeros(0, np.uint32)
+        return cls(o, v)
+
+    def __len__(self) -> int:
+        return len(self.offsets) - 1
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        n = len(self)
+        if i < 0:
+            i += n
+        if not 0 <= i < n:
+            raise IndexError(i)
+        return self.verts[self.offsets[i]:self.offsets[i + 1]]
+
+    def __iter__(self):
+        o, v = self.offsets, self.verts
+        for i in range(len(self)):
+            yield v[o[i]:o[i + 1]]
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def multiset_hash(sets: "SetList") -> str:
+    """Hash of a multiset of vertex sets that ignores the order of the sets
+    and of the vertices in each (a fast equality check between runs, not a
+    cryptographic digest): per set, the sum of a 64-bit mix of every vertex,
+    mixed with the set's size; the sorted per-set values are hashed."""
+    o = np.asarray(sets.offsets, np.int64)
+    n = len(o) - 1
+    if n <= 0:
+        return hashlib.sha256(b"").hexdigest()
+    lens = np.diff(o).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = _mix64(np.asarray(sets.verts, np.uint64))
+        cs = np.concatenate(([np.uint64(0)], np.cumsum(x, dtype=np.uint64)))
+        per = _mix64((cs[o[1:]] - cs[o[:-1]]) ^ _mix64(lens))
+    per.sort()
+    return hashlib.sha256(per.tobytes()).hexdigest()
+
+
 class Graph:
     """An undirected graph held by the library (symmetric CSR, sorted lists)."""
 
@@ -245,7 +309,7 @@ class Result:
     """Maximal sets (ascending, lexicographic order), raw sets and statistics."""
 
     maximal: list[np.ndarray]
-    raw: list[np.ndarray]
+    raw: SetList
     stats: dict
     status: int
     trace_feats: np.ndarray | None = None     # (rows, 12) float32
@@ -273,7 +337,7 @@ def _result_from(h: C.c_void_p, rc: int) -> Result:
         _check(L.qc_result_view(h, C.byref(cnt), C.byref(off), C.byref(v)))
         maximal = _sets_from(cnt.value, off, v)
         _check(L.qc_result_raw(h, C.byref(cnt), C.byref(off), C.byref(v)))
-        raw = _sets_from(cnt.value, off, v)
+        raw = SetList.from_view(cnt.value, off, v)
         st = _Stats()
         _check(L.qc_result_stats(h, C.byref(st)))
         stats = {n: int(getattr(st, n)) for n in _STAT_U64}

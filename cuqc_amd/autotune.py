@@ -65,10 +65,15 @@ class Schedule:
 
 
 def _digest(results) -> str:
+    from . import SetList, multiset_hash
+
     h = hashlib.sha256()
     for r in results:
-        for s in sorted(tuple(sorted(int(x) for x in s)) for s in r.raw):
-            h.update((" ".join(map(str, s)) + "\n").encode())
+        if isinstance(r.raw, SetList):
+            h.update(multiset_hash(r.raw).encode())
+        else:
+            for s in sorted(tuple(sorted(int(x) for x in s)) for s in r.raw):
+                h.update((" ".join(map(str, s)) + "\n").encode())
         h.update(b"|")
     return h.hexdigest()
 
