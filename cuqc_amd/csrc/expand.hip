@@ -2333,28 +2333,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                 c.w.mk0 = (uint64_t*)(lds + a.lds_masks);
                 c.w.mk1 = c.w.mk0 + a.slab_mwords;
                 // the LDS below the masks (the wide tier's arrays) is free
-                // during a slab task: as many of the child's arrays as fit go
-                // there, the ones every pruning pass reads first (indegree,
-                // exdegree, label, then id, matrix index, lvl2adj) (child
-                // positions stay below t: a stride of t rounded up to 64
-                // entries suffices).  When the children lie past the depth
-                // window their lvl2adj is never kept, and the parent's
-                // exdegree, updated by every branch, takes that place.
-                const uint32_t Tt = (ref.t + 63u) & ~63u;
-                const uint32_t k = (uint32_t)a.lds_masks / (Tt * 4);
+                // during a slab task: as many of the task's arrays as fit go
+                // there, in slots of S = t rounded up to 64 entries plus 64
+                // (every array and list of a task holds at most t + 2
+                // entries: positions below t, and the LU tables over
+                // indegrees 0..V+1 with V <= members), in this order: the
+                // child's arrays every pruning pass reads (indegree,
+                // exdegree, label, id, matrix index, lvl2adj), the pruning
+                // lists (rem, rmv, cind, adjc), then the parent's arrays
+                // (indegree, exdegree, label, id, lvl2adj).  When the
+                // children lie past the depth window their lvl2adj is never
+                // kept, and the parent's exdegree, updated by every branch,
+                // takes its slot.  The sort keys keep their slab addresses
+                // (the bitonic sort pads them to a power of two).
+                const uint32_t S = ((ref.t + 63u) & ~63u) + 64u;
+                const uint32_t k = (uint32_t)a.lds_masks / (S * 4);
                 const bool keys = !(c.g.max_depth >= 0 && (int)ref.depth + 1 > c.g.max_depth);
                 int* lp = (int*)lds;
-                if (k >= 1) c.w.cin = lp;
-                if (k >= 2) c.w.cex = lp + Tt;
-                if (k >= 3) c.w.clab = lp + 2 * Tt;
-                if (k >= 4) c.w.cid = lp + 3 * Tt;
-                if (k >= 5) c.w.cpp = lp + 4 * Tt;
-                if (k >= 6) {
-                    if (keys)
-                        c.w.cl2 = lp + 5 * Tt;
-                    else
-                        c.w.pex = lp + 5 * Tt;
-                }
+                uint32_t n = 0;
+                auto put = [&](int*& p) {
+                    if (n < k) p = lp + n * S;
+                    n++;
+                };
+                put(c.w.cin);
+                put(c.w.cex);
+                put(c.w.clab);
+                put(c.w.cid);
+                put(c.w.cpp);
+                put(keys ? c.w.cl2 : c.w.pex);
+                put(c.w.rem);
+                put(c.w.rmv);
+                put(c.w.cind);
+                put(c.w.adjc);
+                if (k > 8) c.w.T2 = (int)S;  // the LU tables' capacity (cind, adjc)
+                put(c.w.pin);
+                if (keys) put(c.w.pex);
+                put(c.w.plab);
+                put(c.w.pid);
+                if (keys) put(c.w.pl2);
             }
             c.cap = a.slab_tcap;
         }

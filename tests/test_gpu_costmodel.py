@@ -157,15 +157,22 @@ def test_cost_order_with_corrupted_model(cfg0_model, kind):
 def test_trained_model_gain_on_held_out_instances():
     """What the trained network gains over the method without it: on traces
     of instances it never saw, its predicted cost ranks the subtrees (rank
-    correlation well above an untrained network's), and cost-ordered runs
-    with it stay within the FIFO run's time while producing the same sets."""
+    correlation well above an untrained network's), and on the batch of the
+    244 held-out instances the cost-ordered launch gives the same sets in
+    about the FIFO launch's kernel time (medians of three runs each).  With
+    the hand-over scheduler the waves rarely run short of work, so ordering
+    by predicted subtree size has little left to gain: the measured change is
+    printed, and the assertion bounds the model's overhead (the features and
+    the forward pass for every child) to 10%."""
+    import statistics
+
     from cuqc_amd import workloads
     w3 = workloads.WORKLOADS[3]
     train = [workloads.graph(w3, i) for i in range(0, 12)]
-    held = [workloads.graph(w3, i) for i in range(12, 24)]
+    held = [workloads.graph(w3, i) for i in range(12, w3.instances)]
     jobs = [(g, g.gamma, g.min_size, w3.window) for g in train]
     model, rows = cm.train_from_traces(jobs, steps=400)
-    res = [qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", window=w3.window, trace=True) for g in held]
+    res = [qc.mine(g, g.gamma, g.min_size, 0, mode="persistent", window=w3.window, trace=True) for g in held[:12]]
     X, y = cm.trace_rows(res)
     assert len(y) > 1000
     rho_trained = _spearman(model.predict(X), y)
@@ -174,13 +181,19 @@ def test_trained_model_gain_on_held_out_instances():
          f"({len(y)} tasks, trained on {rows})")
     assert rho_trained > 0.4
     assert rho_trained > rho_random + 0.2
-    kw = dict(windows=[w3.window] * len(held), split_chunk=-1)
+    kw = dict(windows=[w3.window] * len(held))
     gm, ms = [g.gamma for g in held], [g.min_size for g in held]
-    fifo = qc.mine_batch(held, gm, ms, **kw)
-    cost = qc.mine_batch(held, gm, ms, order="cost", cost_model=model, **kw)
-    for a, b in zip(fifo, cost):
-        assert a.maximal_sorted_tuples() == b.maximal_sorted_tuples()
-        assert a.stats["tasks"] == b.stats["tasks"]
-    tf, tc = fifo[0].stats["t_kernel_ms"], cost[0].stats["t_kernel_ms"]
-    _say(f"[gpu] held-out batch kernel: fifo {tf:.2f} ms, cost-ordered {tc:.2f} ms")
-    assert tc <= 1.5 * tf + 5.0
+    tfs, tcs = [], []
+    for rep in range(3):
+        fifo = qc.mine_batch(held, gm, ms, **kw)
+        cost = qc.mine_batch(held, gm, ms, order="cost", cost_model=model, **kw)
+        if rep == 0:
+            for a, b in zip(fifo, cost):
+                assert a.maximal_sorted_tuples() == b.maximal_sorted_tuples()
+                assert a.stats["tasks"] == b.stats["tasks"]
+        tfs.append(fifo[0].stats["t_kernel_ms"])
+        tcs.append(cost[0].stats["t_kernel_ms"])
+    tf, tc = statistics.median(tfs), statistics.median(tcs)
+    _say(f"[gpu] held-out batch of {len(held)} kernel: fifo {tf:.2f} ms {tfs}, cost-ordered {tc:.2f} ms {tcs}, "
+         f"gain {100 * (1 - tc / tf):.1f}%")
+    assert tc <= 1.10 * tf + 2.0
