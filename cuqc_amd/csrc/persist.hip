@@ -201,8 +201,27 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     const size_t arr_b = wide ? ws_bytes(wide, false, 0) : (size_t)0;
     const size_t mask_b = (size_t)2 * std::max<size_t>((size_t)maxrw, midw) * 8;
     const bool mask_lds = mask_b <= MASK_LDS_MAX;
-    const size_t lds_b = std::max(ws_bytes(TL, true, (TL + 63) / 64), arr_b + (mask_lds ? mask_b : 0));
+    size_t lds_b = std::max(ws_bytes(TL, true, (TL + 63) / 64), arr_b + (mask_lds ? mask_b : 0));
     QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds_b));
+    if (mask_lds && per_cu > 0) {
+        // the resident waves leave part of the CU's LDS unused: the dynamic
+        // LDS grows to the most that keeps per_cu waves resident, and tasks
+        // above the wide tier place arrays there (the masks sit at its end)
+        int lds_cu = 0, lds_blk = 0;
+        QC_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+        QC_HIP(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        hipFuncAttributes fa{};
+        QC_HIP(hipFuncGetAttributes(&fa, kern));
+        const size_t per = (size_t)lds_cu / (size_t)per_cu;
+        size_t cand = per > fa.sharedSizeBytes ? (per - fa.sharedSizeBytes) & ~(size_t)511 : 0;
+        cand = std::min(cand, (size_t)lds_blk > fa.sharedSizeBytes ? (size_t)lds_blk - fa.sharedSizeBytes : (size_t)0);
+        for (; cand > lds_b; cand -= 512) {
+            int pc = 0;
+            QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kern, 64, cand));
+            if (pc >= per_cu) break;
+        }
+        if (cand > lds_b) lds_b = cand;
+    }
     // every wave must be resident: the queue hand-off waits on producers
     uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 16));
     if (p.workgroups > 0) grid = std::min<uint32_t>(grid, (uint32_t)p.workgroups);
@@ -337,7 +356,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.slab_mwords = (uint32_t)maxrw;
         A.mid_tcap = use_mid ? mid : 0;
         A.lds_wide = wide;
-        A.lds_masks = mask_lds ? (int32_t)arr_b : -1;
+        A.lds_masks = mask_lds ? (int32_t)(lds_b - mask_b) : -1;
         A.stats = ws.stats.as<DevStats>();
         A.error = ws.err.as<int>();
         A.queue_mode = 1;
