@@ -204,13 +204,23 @@ __global__ void nm_post_keys(const uint64_t* __restrict__ off, const uint32_t* _
 }
 
 // maximal[u] = 1 iff unique set uniq[u] (u < nu) is not a proper subset of
-// another; 0 for nu <= u < k.  Postings list the largest sets first: the
-// scan stops at the first superset, or at the first set no longer than s.
+// another; 0 for nu <= u < k.  The candidates are the sets posted under the
+// rarest key of s that are longer than s; postings list the longest first,
+// so the scan ends with the first chunk that reaches a set no longer than s.
+// The wave tests 64 candidates at once, one per lane: s sits sorted in LDS,
+// and a lane walks its candidate's sorted keys (eight loads in flight) and
+// merges them against s; a key of s smaller than the candidate's current key
+// is missing from it, which ends the lane's walk.  Sets of more than
+// NM_SUB_LDS keys are tested the other way round: the lanes take s's keys and
+// binary-search each candidate in turn.
+constexpr uint32_t NM_SUB_LDS = 2048;
+
 __global__ __launch_bounds__(64) void nm_subset(const uint64_t* __restrict__ off, const uint32_t* __restrict__ v,
                                                 const uint32_t* __restrict__ uniq, const uint32_t* __restrict__ nu_p,
                                                 uint64_t k, const uint32_t* __restrict__ start,
                                                 const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ post,
                                                 uint32_t* __restrict__ maximal) {
+    __shared__ uint32_t ss[NM_SUB_LDS];
     const int lane = threadIdx.x & 63;
     const uint32_t nu = *nu_p;
     for (uint64_t u = blockIdx.x; u < k; u += gridDim.x) {
@@ -239,24 +249,61 @@ __global__ __launch_bounds__(64) void nm_subset(const uint64_t* __restrict__ off
         }
         bool sub = false;
         const uint32_t pb = start[bx], pe = pb + cnt[bx];
-        for (uint32_t q = pb; q < pe && !sub; q++) {
-            const uint32_t t = post[q];
-            const uint64_t tb = off[t], te = off[t + 1];
-            if (te - tb <= ls) break;
-            bool miss = false;
-            for (uint64_t j = sb + lane; j < se; j += 64) {
-                uint32_t x = v[j];
-                uint64_t lo = tb, hi = te;
-                while (lo < hi) {
-                    uint64_t mid = (lo + hi) >> 1;
-                    if (v[mid] < x)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
+        if (ls <= NM_SUB_LDS) {
+            __syncthreads();  // the previous set's readers are done with ss
+            for (uint64_t j = lane; j < ls; j += 64) ss[j] = v[sb + j];
+            __syncthreads();
+            for (uint32_t q0 = pb; q0 < pe && !sub; q0 += 64) {
+                const uint32_t q = q0 + (uint32_t)lane;
+                uint64_t tb = 0, te = 0;
+                if (q < pe) {
+                    const uint32_t t = post[q];
+                    tb = off[t];
+                    te = off[t + 1];
                 }
-                if (lo == te || v[lo] != x) miss = true;
+                const bool live = q < pe && te - tb > ls;
+                // a set no longer than s in this chunk: none further on is longer
+                const bool last = __ballot(q < pe && !live) != 0;
+                uint64_t i = 0;
+                bool fail = !live;
+                for (uint64_t j0 = tb; !fail && i < ls; j0 += 8) {
+                    uint32_t e[8];
+#pragma unroll
+                    for (int x = 0; x < 8; x++) e[x] = j0 + x < te ? v[j0 + x] : 0xFFFFFFFFu;
+#pragma unroll
+                    for (int x = 0; x < 8; x++) {
+                        if (!fail && i < ls) {
+                            const uint32_t y = ss[i];
+                            if (y < e[x])
+                                fail = true;  // y is not in the candidate (past te: e = ~0)
+                            else if (y == e[x])
+                                i++;
+                        }
+                    }
+                }
+                sub = __ballot(!fail && i == ls) != 0;
+                if (last) break;
             }
-            sub = __ballot(miss) == 0;
+        } else {
+            for (uint32_t q = pb; q < pe && !sub; q++) {
+                const uint32_t t = post[q];
+                const uint64_t tb = off[t], te = off[t + 1];
+                if (te - tb <= ls) break;
+                bool miss = false;
+                for (uint64_t j = sb + lane; j < se; j += 64) {
+                    uint32_t x = v[j];
+                    uint64_t lo = tb, hi = te;
+                    while (lo < hi) {
+                        uint64_t mid = (lo + hi) >> 1;
+                        if (v[mid] < x)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    if (lo == te || v[lo] != x) miss = true;
+                }
+                sub = __ballot(miss) == 0;
+            }
         }
         if (lane == 0) maximal[u] = sub ? 0u : 1u;
     }

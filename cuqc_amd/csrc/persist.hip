@@ -125,6 +125,10 @@ int ensure(DBuf& b, size_t bytes) {
 constexpr int MID_MAX = 4096;
 // largest pair of set masks kept in LDS (more would cost resident waves)
 constexpr size_t MASK_LDS_MAX = 4096;
+// LDS allocation granule of a workgroup on gfx950 (bytes), measured: single-
+// wave workgroups of 32000 B run 5 per 160 KB CU, of 32256 B 4; of 20480 B
+// 8, of 20736 B 7
+constexpr size_t LDS_GRANULE = 1280;
 }  // namespace
 
 int enumerate_persistent(const Instance* const* insts, const qc_params* params, int count, EnumOut* outs,
@@ -203,24 +207,34 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     const bool mask_lds = mask_b <= MASK_LDS_MAX;
     size_t lds_b = std::max(ws_bytes(TL, true, (TL + 63) / 64), arr_b + (mask_lds ? mask_b : 0));
     QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds_b));
-    if (mask_lds && per_cu > 0) {
-        // the resident waves leave part of the CU's LDS unused: the dynamic
-        // LDS grows to the most that keeps per_cu waves resident, and tasks
-        // above the wide tier place arrays there (the masks sit at its end)
+    {
+        // A workgroup's LDS (static + dynamic) is allocated in granules of
+        // LDS_GRANULE bytes, coarser than the occupancy API assumes: near a
+        // boundary the API counts one more resident wave than the CU holds
+        // (160 KB: 5 workgroups of 32000 B fit, of 32256 B only 4).  Waves
+        // that are not resident would only start when others leave, and the
+        // static order deals positions to every wave, so the grid must not
+        // count them.
         int lds_cu = 0, lds_blk = 0;
         QC_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
         QC_HIP(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         hipFuncAttributes fa{};
         QC_HIP(hipFuncGetAttributes(&fa, kern));
-        const size_t per = (size_t)lds_cu / (size_t)per_cu;
-        size_t cand = per > fa.sharedSizeBytes ? (per - fa.sharedSizeBytes) & ~(size_t)511 : 0;
-        cand = std::min(cand, (size_t)lds_blk > fa.sharedSizeBytes ? (size_t)lds_blk - fa.sharedSizeBytes : (size_t)0);
-        for (; cand > lds_b; cand -= 512) {
+        const size_t st = fa.sharedSizeBytes;
+        auto alloc = [&](size_t dyn) { return (dyn + st + LDS_GRANULE - 1) / LDS_GRANULE * LDS_GRANULE; };
+        per_cu = std::min(per_cu, (int)((size_t)lds_cu / alloc(lds_b)));
+        if (mask_lds && per_cu > 0) {
+            // the resident waves leave part of the CU's LDS unused: the
+            // dynamic region grows to the most that keeps per_cu waves
+            // resident, and tasks above the wide tier place arrays there
+            // (the masks sit at its end)
+            size_t cand = (size_t)lds_cu / (size_t)per_cu / LDS_GRANULE * LDS_GRANULE;
+            cand = std::min(cand, (size_t)lds_blk);
+            cand = cand > st ? (cand - st) & ~(size_t)255 : 0;
             int pc = 0;
-            QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kern, 64, cand));
-            if (pc >= per_cu) break;
+            if (cand > lds_b) QC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kern, 64, cand));
+            if (cand > lds_b && pc >= per_cu && (size_t)lds_cu / alloc(cand) >= (size_t)per_cu) lds_b = cand;
         }
-        if (cand > lds_b) lds_b = cand;
     }
     // every wave must be resident: the queue hand-off waits on producers
     uint32_t grid = (uint32_t)std::max(1, n_cu) * (uint32_t)std::max(1, std::min(per_cu, 16));
