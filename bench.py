@@ -41,9 +41,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel (expand_persistent) in the
 # headline form, keyed by workload label: one rocprofv3 FETCH_SIZE and one
 # WRITE_SIZE pass, each scaled by the counter's reading for a 256 MiB device
-# copy (FETCH x2.0, WRITE x1.0; profiles/hbm_traffic_*.md).  None where not
-# measured.
-TRAFFIC: dict = {}
+# copy (FETCH x2.000, WRITE x1.000; profiles/expand_persistent_*.md).  The
+# autotuner may pick another schedule on another run; the figure belongs to
+# the profiled one.  None where not measured.
+TRAFFIC: dict = {
+    # 739,970 KiB x 2.000 + 22,662,905 KiB x 1.000
+    "coexpr8k[3600:3612,depth<=3]": 739970 * 1024 * 2 + 22662905 * 1024,
+}
 
 
 def _log(msg: str) -> None:
