@@ -1244,7 +1244,7 @@ __device__ __attribute__((always_inline)) bool push_ref(Ctx& c, const TaskRef& r
             if (ring_claim(c, rg, 1, &pos) == 1) {
                 const unsigned long long gi = (unsigned long long)rg * a.q_cap + pos % a.q_cap;
                 if (c.lane == 0) {
-                    __hip_atomic_fetch_add((unsigned long long*)a.pending, 1ull, __ATOMIC_RELAXED,
+                    __hip_atomic_fetch_add(a.pending, 1ull, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
                     a.q_refs[gi] = r;
                     if (a.trace_parent) a.trace_parent[gi] = r.b1 > 0 ? -2 : c.slot;
@@ -1286,7 +1286,7 @@ __device__ __attribute__((always_inline)) bool static_flush(Ctx& c) {
         if (c.lane < k)
             a.q_refs[(pos + c.lane) % a.q_cap] = c.pstack[(c.pbot + (uint32_t)c.lane) % a.priv_cap];
         if (c.lane == 0)
-            __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         ring_publish(c, 0, pos, k);
         c.pbot += (uint32_t)k;
@@ -1756,7 +1756,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
             const int k = ring_claim(c, rg, min(n_items - done, WAVE), &pos);
             if (k == 0) break;
             if (c.lane == 0)
-                __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(a.pending, (unsigned long long)k, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             if (c.lane < k) {
                 const unsigned long long gi = (unsigned long long)rg * a.q_cap + (pos + c.lane) % a.q_cap;
@@ -2102,13 +2102,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                 // pending - (G - hungry) tasks; handing over more than the
                 // waiting waves take turns the search breadth first and
                 // multiplies the live records.
-                long long hg = 0, pd = 0;
-                if (c.lane == 0) {
-                    hg = __hip_atomic_load(a.hungry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    pd = __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                hg = (long long)rdlane64((unsigned long long)hg, 0);
-                pd = (long long)rdlane64((unsigned long long)pd, 0);
+                unsigned long long ph = 0;
+                if (c.lane == 0) ph = __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ph = rdlane64(ph, 0);
+                const long long hg = (long long)(ph >> 32), pd = (long long)(ph & 0xFFFFFFFFull);
                 const long long waiting = pd - ((long long)G - hg);
                 const long long need = hg - (waiting > 0 ? waiting : 0);
                 if (need > 0) {
@@ -2150,7 +2147,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                             a.q_refs[(unsigned long long)rg * a.q_cap + (pos + c.lane) % a.q_cap] =
                                 c.pstack[(c.pbot + (uint32_t)c.lane) % a.priv_cap];
                         if (c.lane == 0)
-                            __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)k,
+                            __hip_atomic_fetch_add(a.pending, (unsigned long long)k,
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ring_publish(c, rg, pos, k);
                         c.pbot += (uint32_t)k;
@@ -2237,7 +2234,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             // relaxed: an acquire invalidates the CU's vector L1 under the
             // waves still working.
             if (c.lane == 0 && state == 0 &&
-                __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+                (__hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0xFFFFFFFFull) == 0)
                 state = 2;
             state = rdlane(state, 0);
             if (state == 2) break;
@@ -2247,9 +2244,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                     // out of work: no longer counted in pending, now hungry
                     busy = false;
                     if (c.lane == 0) {
-                        __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)-1ll,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        __hip_atomic_fetch_add(a.hungry, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(a.pending, PH_HUNGRY - 1ull, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                     }
                     continue;  // look once more before sleeping
                 }
@@ -2273,10 +2269,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             bsel = rdlane(bsel, 0);
             // the claimed tasks leave the rings; an idle wave becomes busy
             if (c.lane == 0) {
-                const long long d = (busy ? 0 : 1) - (long long)nk;
-                if (d) __hip_atomic_fetch_add((unsigned long long*)a.pending, (unsigned long long)d, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-                if (!busy) __hip_atomic_fetch_add(a.hungry, -1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long d =
+                    (unsigned long long)((busy ? 0 : 1) - (long long)nk) - (busy ? 0ull : PH_HUNGRY);
+                if (d) __hip_atomic_fetch_add(a.pending, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             busy = true;
             // one acquire per claim (pairs with the producers' release before

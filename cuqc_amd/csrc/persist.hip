@@ -27,7 +27,7 @@ __global__ void expand_persistent(const ExpandArgs*);
 // Queue state for a fresh run: the roots dealt over the top bucket's ns
 // rings (root i at position i / ns of shard i mod ns, published), every
 // other cell free for its first lap, the free lists empty, counters reset.
-// ctr = heads[nr], tails[nr], pool top, pending, overflow, hungry (nr =
+// ctr = heads[nr], tails[nr], pool top, hungry|pending, overflow (nr =
 // nb ns rings), QSTRIDE words apart.  Root records sit at the start of the
 // pool; allocated blocks start at the next unit boundary.
 __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots, const Vrec* __restrict__ root_verts,
@@ -62,9 +62,8 @@ __global__ void persist_reset(const TaskRef* __restrict__ roots, uint32_t nroots
             ctr[(nr + r) * QSTRIDE] = r >= top && r - top < nroots ? (nroots - (r - top) + ns - 1) / ns : 0;
         }
         ctr[2 * nr * QSTRIDE] = (root_slots + POOL_UNIT - 1) / POOL_UNIT * POOL_UNIT;  // pool top
-        ctr[(2 * nr + 1) * QSTRIDE] = nroots;                 // pending
+        ctr[(2 * nr + 1) * QSTRIDE] = (uint64_t)n_waves << 32 | nroots;  // hungry: every wave, pending: the roots
         ctr[(2 * nr + 2) * QSTRIDE] = 0;                      // overflow
-        ctr[(2 * nr + 3) * QSTRIDE] = n_waves;                // hungry: every wave starts without work
         clq_ctr->items = 0;
         clq_ctr->slots = 0;
         *stats = DevStats{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -86,7 +85,7 @@ size_t ws_bytes_host(uint32_t tcap, bool mat, uint32_t mwords) {
 
 constexpr int Q_SHARDS = 8;                 // rings per bucket in the dynamic order: one per XCD
 constexpr int NCTR = 2 * 4 * Q_SHARDS + 4;  // heads, tails (4 buckets x Q_SHARDS rings at most), pool top,
-                                            // pending, overflow, hungry
+                                            // hungry|pending, overflow, one spare
 
 struct RunState {
     unsigned long long ctr[NCTR];
@@ -393,9 +392,8 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.q_head = ctr;
         A.q_tail = ctr + nr * QSTRIDE;
         A.pool_top = ctr + 2 * nr * QSTRIDE;
-        A.pending = (long long*)(ctr + (2 * nr + 1) * QSTRIDE);
+        A.pending = ctr + (2 * nr + 1) * QSTRIDE;
         A.overflow = ctr + (2 * nr + 2) * QSTRIDE;
-        A.hungry = (long long*)(ctr + (2 * nr + 3) * QSTRIDE);
         A.n_buckets = nb;
         A.q_shards = ns;
         A.n_roots = (int32_t)roots.size();

@@ -20,6 +20,7 @@ constexpr int QSTRIDE = 32;
 // a free list of its class and is reused by the next record of that class.
 constexpr uint32_t POOL_UNIT = 16;
 constexpr int POOL_NCLASS = 112;
+constexpr unsigned long long PH_HUNGRY = 1ull << 32;  // one hungry wave in ExpandArgs::pending
 constexpr int FL_SHARDS = 16;     // shared free lists per class (a wave uses blockIdx % FL_SHARDS)
 constexpr int FL_SMALL = 8;       // classes with per-wave LDS stacks (1..8 units)
 constexpr int FL_LOCAL_MAX = 8;   // LDS stack entries per small class
@@ -192,9 +193,13 @@ struct ExpandArgs {
                                      // lines of QSTRIDE words: (tag << 36) | head block index
                                      // (POOL_UNIT-Vrec units), index all ones = empty
     uint32_t fl_mag;                 // blocks per magazine of the small classes (1..FL_LOCAL_MAX / 2)
-    long long* pending;
+    // (hungry << 32) | pending in one word, so a wave's transition between
+    // holding work and waiting is one atomic: `pending` counts the tasks in
+    // the rings plus the waves holding work, `hungry` the waves holding
+    // none (they ask the busy ones to hand some over).  Both stay below 2^31
+    // and never go negative: every decrement follows its increment.
+    unsigned long long* pending;
     unsigned long long* overflow;    // pushes that found the ring and the private stack full
-    long long* hungry;               // waves holding no work (they ask the busy ones to hand some over)
     // private stacks: wave w owns priv[w priv_cap, (w+1) priv_cap) (refs)
     // and priv_bkt[...] (the bucket each ref would take in the rings);
     // children stay there (depth first) unless use_priv is 0, and the
