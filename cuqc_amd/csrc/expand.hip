@@ -1326,8 +1326,12 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
     int s = (uint32_t)tc <= a.small_max ? 0 : 1;
     int bucket = 0;
     float feat[12];
-    if (q && (a.trace_feat || a.n_buckets > 1)) task_features(c, tc, mem, (int)c.depth + 1, gap, feat);
-    if (q && a.n_buckets > 1) bucket = cost_bucket(c, feat);
+    // cost order: children that fit the LDS tile take the lowest bucket
+    // without the model (their subtrees are small, and on trees of tiny
+    // tasks the features and the forward pass would cost as much as the task)
+    const bool predict = q && a.n_buckets > 1 && (uint32_t)tc > a.cost_min_t;
+    if (q && (a.trace_feat || predict)) task_features(c, tc, mem, (int)c.depth + 1, gap, feat);
+    if (predict) bucket = cost_bucket(c, feat);
     unsigned long long ri = 0, si = 0;
     if (q) ph_mark(c, PH_CHILD);
     if (c.lane == 0) {

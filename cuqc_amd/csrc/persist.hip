@@ -375,6 +375,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.queue_mode = 1;
         A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 64 : p.split_chunk);
         A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : 512);
+        A.cost_min_t = (uint32_t)TL;
         A.q_refs = ws.q.as<TaskRef>();
         A.q_seq = ws.seq.as<unsigned long long>();
         A.fl_head = ws.fl.as<unsigned long long>();

@@ -181,6 +181,7 @@ struct ExpandArgs {
     int32_t queue_mode;
     int32_t split_chunk;             // branches per split item (0: no splitting)
     uint32_t split_min_t;            // split tasks above this size
+    uint32_t cost_min_t;             // cost order: children of at most this size skip the model (bucket 0)
     TaskRef* q_refs;
     unsigned long long* q_seq;
     unsigned long long* q_head;
