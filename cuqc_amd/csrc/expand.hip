@@ -244,6 +244,8 @@ struct Ctx {
     // persistent mode: this wave's private stack (entries [pbot, ptop) of a
     // ring of priv_cap refs; uniform counters, lane 0 touches the entries)
     int shard;                        // this wave's ring shard (blockIdx % q_shards)
+    TaskRef* ptopref;                 // LDS: a copy of stack entry ptopi (the last pushed), ~0u: none
+    uint32_t ptopi;
     uint32_t* fl;                     // LDS: free block indices of the small classes, FL_LOCAL_MAX per class
     uint32_t* fln;                    // LDS: their counts
     TaskRef* pstack;
@@ -1174,15 +1176,17 @@ __device__ __attribute__((always_inline)) void pool_free(Ctx& c, unsigned long l
     pool_free_shared(a, c.fl, c.fln, cl, idx);
 }
 
-// Claims up to n (<= 64) consecutive free cells of bucket b for producers
+// Claims up to n (<= 64) consecutive free cells of ring b for producers
 // (whole wave; lane j checks the cell of position pos + j).  Returns how
-// many were claimed (0: the ring is full), their first position in *pos.
-// With a trace the ring must not wrap (positions index the trace rows).
-__device__ __attribute__((always_inline)) int ring_claim(Ctx& c, int b, int n, unsigned long long* pos_out) {
+// many were claimed (0: the ring is full, or max_tries attempts lost to
+// other producers), their first position in *pos.  With a trace the ring
+// must not wrap (positions index the trace rows).
+__device__ __attribute__((always_inline)) int ring_claim(Ctx& c, int b, int n, unsigned long long* pos_out,
+                                                        int max_tries = 1 << 16) {
     const ExpandArgs& a = *c.a;
     unsigned long long* tp = &a.q_tail[b * QSTRIDE];
     const unsigned long long* seq = a.q_seq + (size_t)b * a.q_cap;
-    for (int tries = 0; tries < (1 << 16); tries++) {
+    for (int tries = 0; tries < max_tries; tries++) {
         unsigned long long pos = 0;
         if (c.lane == 0) pos = __hip_atomic_load(tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         pos = rdlane64(pos, 0);
@@ -1258,7 +1262,9 @@ __device__ __attribute__((always_inline)) bool push_ref(Ctx& c, const TaskRef& r
             if (c.lane == 0) {
                 c.pstack[c.ptop % a.priv_cap] = r;
                 c.pbkt[c.ptop % a.priv_cap] = (unsigned char)bkt;
+                *c.ptopref = r;  // the entry the next pop takes, kept in LDS too
             }
+            c.ptopi = c.ptop;
             c.ptop++;
             return true;
         }
@@ -2080,6 +2086,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     // positions claimed from a ring and not yet run: their refs wait in LDS
     // (registers are the scarce resource of this kernel)
     __shared__ TaskRef s_claim[4];
+    __shared__ TaskRef s_top;
+    c.ptopref = &s_top;
+    c.ptopi = ~0u;
     int nq = 0, qi = 0;
     unsigned long long qbase = 0, qh = 0;
     // `pending` counts the tasks waiting in the rings plus the waves that hold
@@ -2092,6 +2101,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
         long long gslot;
         if (qi < nq) {
             ref = s_claim[qi];
+            if (c.phase_on && c.lane == 0) c.ph[PH_N_RING]++;
             gslot = (long long)(qbase + (qh + (unsigned long long)qi) % a.q_cap);
             qi++;
         } else if (a.static_sched && c.ptop != c.pbot && static_flush(c)) {
@@ -2099,6 +2109,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
         } else if (c.ptop != c.pbot) {
             // this wave's own stack (depth first)
             idle = 0;
+            ph_mark(c, PH_POP);
             if (!a.static_sched && c.ptop - c.pbot > 1) {
                 // more waves wait for work than the rings hold: hand over
                 // the oldest (shallowest, so largest) records, up to 16 at
@@ -2141,8 +2152,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                     int bk = 0;
                     if (c.lane == 0) bk = c.pbkt[c.pbot % a.priv_cap];
                     const int rg = rdlane(bk, 0) * a.q_shards + c.shard;
+                    // a hand-over is optional: one attempt, and when another
+                    // producer moved the tail first the wave keeps its records
+                    // (retrying under contention cost more than the records
+                    // were worth on trees of tiny tasks)
                     unsigned long long pos = 0;
-                    const int k = ring_claim(c, rg, want, &pos);
+                    const int k = ring_claim(c, rg, want, &pos, 2);
                     if (k > 0) {
                         // lane j copies entry j (the barrier makes lane 0's
                         // stack stores visible to the other lanes)
@@ -2155,14 +2170,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ring_publish(c, rg, pos, k);
                         c.pbot += (uint32_t)k;
+                        if (c.phase_on && c.lane == 0) c.ph[PH_N_GIVEN] += (unsigned long long)k;
                     }
                 }
             }
+            ph_mark(c, PH_GIVE);
             c.ptop--;
             TaskRef r;
-            if (c.lane == 0) r = c.pstack[c.ptop % a.priv_cap];
+            // the newest entry is usually still in LDS (pushed last)
+            if (c.lane == 0) r = c.ptopi == c.ptop ? *c.ptopref : c.pstack[c.ptop % a.priv_cap];
+            c.ptopi = ~0u;
             ref = bcast_ref(r, 0);
             gslot = -1;
+            if (c.phase_on && c.lane == 0) c.ph[PH_N_OWN]++;
         } else {
             // ---- the shared rings ----
             int state = 0;  // 0 nothing published, 1 got positions, 2 done, 3 retry now
@@ -2281,13 +2301,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             // one acquire per claim (pairs with the producers' release before
             // the sequence words), then lane q reads the ref of position h + q
             // and gives the cell back for the next lap
+            ph_mark(c, PH_POP);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             const unsigned long long base = (unsigned long long)bsel * a.q_cap;
             if (c.lane < nk) s_claim[c.lane] = a.q_refs[base + (h + c.lane) % a.q_cap];
             wsync();
+            // the refs are in LDS (the barrier waited for their loads), so
+            // the cells go back with plain relaxed stores: no release, whose
+            // agent-scope form writes back this XCD's L2
             if (c.lane < nk)
                 __hip_atomic_store(&a.q_seq[base + (h + c.lane) % a.q_cap], h + (unsigned long long)c.lane + a.q_cap,
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ph_mark(c, PH_SYNC);
             nq = nk;
             qi = 0;
             qbase = base;

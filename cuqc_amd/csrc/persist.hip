@@ -555,13 +555,14 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         unsigned long long ph[PH_COUNT];
         QC_HIP(hipMemcpy(ph, d_phase.p, sizeof(ph), hipMemcpyDeviceToHost));
         static const char* names[] = {"load", "lookahead", "remove", "add", "prune", "child", "prepare", "idle",
-                                      "lu", "degupd", "classify", "critical", "compact", "hist", "pool", "pop"};
+                                      "lu", "degupd", "classify", "critical", "compact", "hist", "pool", "pop", "give", "sync"};
         double tot = 0;
-        for (int k = 0; k < PH_COUNT; k++) tot += (double)ph[k];
+        for (int k = 0; k < PH_CYCLES; k++) tot += (double)ph[k];
         fprintf(stderr, "[qcmine] phases (grid %u waves, %.3g wave-cycles):", grid, tot);
-        for (int k = 0; k < PH_COUNT; k++)
+        for (int k = 0; k < PH_CYCLES; k++)
             fprintf(stderr, " %s %.1f%%", names[k], 100.0 * ph[k] / (tot > 0 ? tot : 1));
-        fprintf(stderr, "\n");
+        fprintf(stderr, "; tasks from own stacks %llu, from rings %llu, handed over %llu\n", ph[PH_N_OWN],
+                ph[PH_N_RING], ph[PH_N_GIVEN]);
     }
 
     // per-instance outputs (the last replay's buffers; replays are identical

@@ -250,6 +250,12 @@ enum PhaseId {
     PH_MOVE,          // degree-pruning compaction of the kept candidates
     PH_HIST,          // candidate indegree tables for the LU bounds
     PH_POOL,          // persistent form: record blocks taken and given back, task refs pushed
-    PH_POP,           // persistent form: taking the next task (own stack, hand-over, rings)
+    PH_POP,           // persistent form: taking the next task (own stack, rings)
+    PH_GIVE,          // persistent form: deciding on and making hand-overs from the own stack
+    PH_SYNC,          // persistent form: a ring claim's acquire and the cells' return
+    PH_CYCLES,        // (the entries above count cycles, the ones below events)
+    PH_N_OWN = PH_CYCLES,  // tasks taken from the wave's own stack
+    PH_N_RING,        // tasks taken from the rings
+    PH_N_GIVEN,       // records handed over to the rings
     PH_COUNT
 };
