@@ -2125,28 +2125,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                 const long long need = hg - (waiting > 0 ? waiting : 0);
                 if (need > 0) {
                     const int want = (int)min<long long>(min<long long>((long long)(c.ptop - c.pbot - 1), 16ll), need);
-                    if (a.n_buckets > 1 && c.lane == 0) {
-                        // cost order: among the 16 oldest records, hand over
-                        // the ones the model predicts largest first
-                        const uint32_t span = min<uint32_t>(c.ptop - c.pbot - 1, 16u);
-                        for (int d = 0; d < want; d++) {
-                            const uint32_t at = (c.pbot + (uint32_t)d) % a.priv_cap;
-                            uint32_t best = at;
-                            int bb = c.pbkt[at];
-                            for (uint32_t e = (uint32_t)d + 1; e < span; e++) {
-                                const uint32_t x = (c.pbot + e) % a.priv_cap;
-                                if (c.pbkt[x] > bb) {
-                                    bb = c.pbkt[x];
-                                    best = x;
-                                }
+                    if (a.n_buckets > 1) {
+                        // cost order: the 16 oldest records are ordered by
+                        // predicted bucket, largest first (stable), so the
+                        // hand-over takes the ones the model predicts largest.
+                        // Lane j holds record j; its rank is the number of
+                        // records before it in that order.
+                        wsync();  // lane 0's stack stores, visible to every lane
+                        const int span = (int)min<uint32_t>(c.ptop - c.pbot - 1, 16u);
+                        const uint32_t at = (c.pbot + (uint32_t)c.lane) % a.priv_cap;
+                        const int bj = c.lane < span ? (int)c.pbkt[at] : -1;
+                        int rank = 0;
+                        for (int i = 0; i < span; i++) {
+                            const int bi = rdlane(bj, i);
+                            rank += (bi > bj || (bi == bj && i < c.lane)) ? 1 : 0;
+                        }
+                        if (ballot(c.lane < span && rank != c.lane)) {
+                            TaskRef tr;
+                            if (c.lane < span) tr = c.pstack[at];
+                            wsync();
+                            if (c.lane < span) {
+                                const uint32_t to = (c.pbot + (uint32_t)rank) % a.priv_cap;
+                                c.pstack[to] = tr;
+                                c.pbkt[to] = (unsigned char)bj;
                             }
-                            if (best != at) {
-                                const TaskRef tr = c.pstack[at];
-                                c.pstack[at] = c.pstack[best];
-                                c.pstack[best] = tr;
-                                c.pbkt[best] = c.pbkt[at];
-                                c.pbkt[at] = (unsigned char)bb;
-                            }
+                            wsync();
                         }
                     }
                     int bk = 0;
@@ -2211,9 +2214,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
                 }
                 const uint64_t m = ballot(c.lane < NR && tl > hl);
                 if (m) {
-                    // the highest bucket with work; in it this wave's shard,
-                    // else the next shard after it that has work
-                    const int b = (63 - __builtin_clzll(m)) / NS;
+                    // this wave's own shard first (its XCD's L2 holds the
+                    // records its neighbours wrote), the highest bucket with
+                    // work there; else the highest bucket with work anywhere,
+                    // from the next shard after this one
+                    uint64_t own = 0;
+                    for (int q = 0; q < a.n_buckets; q++) own |= 1ull << (q * NS + c.shard);
+                    const uint64_t mo = m & own;
+                    const int b = (63 - __builtin_clzll(mo ? mo : m)) / NS;
                     const uint64_t full = NS == 64 ? ~0ull : (1ull << NS) - 1;
                     const uint64_t sm = (m >> (b * NS)) & full;
                     const uint64_t rot = ((sm >> c.shard) | (sm << ((NS - c.shard) & 63))) & full;
