@@ -251,6 +251,11 @@ struct Ctx {
     TaskRef* pstack;
     unsigned char* pbkt;
     uint32_t pbot, ptop;
+    // children given straight to the rings while waves wait (share_child)
+    unsigned long long share_v;       // lane 0: the pending word as read at the last check
+    int share_n;                      // children since the last check
+    int share_left;                   // children still to give before the next check
+    bool share_ok;                    // the running task has SHARE_MIN_T vertices or more
 };
 
 // phase accounting: charge the cycles since the last mark to phase k
@@ -1234,6 +1239,39 @@ __device__ __forceinline__ void ring_publish(Ctx& c, int b, unsigned long long p
     }
 }
 
+// Dynamic order: whether this child goes to a ring instead of the wave's own
+// stack.  The stack's records are handed over only between tasks, so while a
+// long task runs (a root, a large first-level task) its children would wait
+// on this wave while other waves have nothing.  In tasks of SHARE_MIN_T
+// vertices or more, every SHARE_EVERY children the wave compares the waves
+// that wait for work with the tasks the rings hold (the pending word, as the
+// hand-over between tasks does) and gives up to 16 of its next children to
+// the rings when they fall short.  Smaller tasks end soon enough for the
+// hand-over between tasks, and sending their children through the rings
+// costs more than the wait (trees of 30-vertex tasks ran slower).  The word
+// is loaded at one check and read at the next, so its latency overlaps the
+// children built in between; a count one interval old only delays the answer.
+constexpr int SHARE_EVERY = 8;
+constexpr uint32_t SHARE_MIN_T = 64;
+__device__ __forceinline__ bool share_child(Ctx& c) {
+    const ExpandArgs& a = *c.a;
+    if (!c.share_ok) return false;
+    if (c.share_left > 0) {
+        c.share_left--;
+        return true;
+    }
+    if (++c.share_n < SHARE_EVERY) return false;
+    c.share_n = 0;
+    const unsigned long long ph = rdlane64(c.share_v, 0);
+    if (c.lane == 0) c.share_v = __hip_atomic_load(a.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long hg = (long long)(ph >> 32), pd = (long long)(ph & 0xFFFFFFFFull);
+    const long long waiting = pd - ((long long)gridDim.x - hg);
+    const long long need = hg - (waiting > 0 ? waiting : 0);
+    if (need <= 0) return false;
+    c.share_left = (int)min<long long>(need, 16ll) - 1;
+    return true;
+}
+
 // One task ref into a ring (shared = true first; counted in `pending` before
 // anyone can take it) or this wave's private stack (covered by the wave's
 // own count while it is busy).  Returns false when both are full (the launch
@@ -1379,7 +1417,7 @@ __device__ __attribute__((always_inline)) void write_child(Ctx& c, int tc, int m
         // static order: the stack only stages them, and the wave moves them
         // to the ring (whose positions are dealt to the waves in turn) in
         // one claim when the task ends
-        const bool shared = !a.use_priv;
+        const bool shared = !a.use_priv || (!a.static_sched && share_child(c));
         const bool pushed = push_ref(c, r, bucket, shared, feat);
         ph_mark(c, PH_POOL);
         if (!pushed) return;
@@ -1454,42 +1492,38 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         c.M1 = w.M1;
         c.M2 = w.M2;
         const int rw = c.g.p->rw;
+        const uint64_t* __restrict__ A1 = c.g.p->A1;
+        const uint64_t* __restrict__ A2 = c.g.p->A2;
+        // lane = column; the rows' words holding the columns' bits are
+        // gathered RU rows at a time (2 RU loads in flight per lane: tasks
+        // here are small, and the build is a few rounds of memory latency);
+        // the last group masks the rows past t0
+        constexpr int RU = 8;
         for (int wd = 0; wd < W; wd++) {
             int col = wd * 64 + c.lane;
             bool cv = col < t0;
             int cb = cv ? w.pid[col] : 0;
             int cw = cb >> 6;
             uint64_t cm = 1ull << (cb & 63);
-            int i = 0;
-            for (; i + 4 <= t0; i += 4) {
-                int r0 = w.pid[i], r1 = w.pid[i + 1], r2 = w.pid[i + 2], r3 = w.pid[i + 3];
-                uint64_t x0 = c.g.p->A1[(size_t)r0 * rw + cw], x1 = c.g.p->A1[(size_t)r1 * rw + cw];
-                uint64_t x2 = c.g.p->A1[(size_t)r2 * rw + cw], x3 = c.g.p->A1[(size_t)r3 * rw + cw];
-                uint64_t y0 = c.g.p->A2[(size_t)r0 * rw + cw], y1 = c.g.p->A2[(size_t)r1 * rw + cw];
-                uint64_t y2 = c.g.p->A2[(size_t)r2 * rw + cw], y3 = c.g.p->A2[(size_t)r3 * rw + cw];
-                uint64_t b0 = ballot(cv && (x0 & cm)), b1 = ballot(cv && (x1 & cm));
-                uint64_t b2 = ballot(cv && (x2 & cm)), b3 = ballot(cv && (x3 & cm));
-                uint64_t e0 = ballot(cv && (y0 & cm)), e1 = ballot(cv && (y1 & cm));
-                uint64_t e2 = ballot(cv && (y2 & cm)), e3 = ballot(cv && (y3 & cm));
-                if (c.lane == 0) {
-                    w.M1[(size_t)(i + 0) * W + wd] = b0;
-                    w.M1[(size_t)(i + 1) * W + wd] = b1;
-                    w.M1[(size_t)(i + 2) * W + wd] = b2;
-                    w.M1[(size_t)(i + 3) * W + wd] = b3;
-                    w.M2[(size_t)(i + 0) * W + wd] = e0;
-                    w.M2[(size_t)(i + 1) * W + wd] = e1;
-                    w.M2[(size_t)(i + 2) * W + wd] = e2;
-                    w.M2[(size_t)(i + 3) * W + wd] = e3;
+            for (int i = 0; i < t0; i += RU) {
+                uint64_t x[RU], y[RU];
+#pragma unroll
+                for (int q = 0; q < RU; q++) {
+                    x[q] = 0;
+                    y[q] = 0;
+                    if (i + q < t0) {
+                        const size_t rb = (size_t)w.pid[i + q] * rw + cw;
+                        x[q] = A1[rb];
+                        y[q] = A2[rb];
+                    }
                 }
-            }
-            for (; i < t0; i++) {
-                int r0 = w.pid[i];
-                uint64_t x0 = c.g.p->A1[(size_t)r0 * rw + cw];
-                uint64_t y0 = c.g.p->A2[(size_t)r0 * rw + cw];
-                uint64_t b0 = ballot(cv && (x0 & cm)), e0 = ballot(cv && (y0 & cm));
-                if (c.lane == 0) {
-                    w.M1[(size_t)i * W + wd] = b0;
-                    w.M2[(size_t)i * W + wd] = e0;
+#pragma unroll
+                for (int q = 0; q < RU; q++) {
+                    const uint64_t b = ballot(cv && (x[q] & cm)), e = ballot(cv && (y[q] & cm));
+                    if (c.lane == 0 && i + q < t0) {
+                        w.M1[(size_t)(i + q) * W + wd] = b;
+                        w.M2[(size_t)(i + q) * W + wd] = e;
+                    }
                 }
             }
         }
@@ -2089,6 +2123,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
     __shared__ TaskRef s_top;
     c.ptopref = &s_top;
     c.ptopi = ~0u;
+    c.share_v = 0;  // reads as nobody waiting
+    c.share_n = 0;
+    c.share_left = 0;
+    c.share_ok = false;
     int nq = 0, qi = 0;
     unsigned long long qbase = 0, qh = 0;
     // `pending` counts the tasks waiting in the rings plus the waves that hold
@@ -2338,6 +2376,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ex
             sinst = ref.inst;
         }
         c.slot = ref.b1 > 0 ? (long long)ref.bound : gslot;
+        c.share_ok = ref.t >= SHARE_MIN_T;
         // workspace: LDS with local matrices, LDS arrays with root
         // matrices, slab with local matrices, or slab in root space
         const bool fits = ref.t <= a.slab_tcap;

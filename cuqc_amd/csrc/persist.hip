@@ -374,11 +374,11 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         A.error = ws.err.as<int>();
         A.queue_mode = 1;
         A.split_chunk = p.split_chunk < 0 ? 0 : (p.split_chunk == 0 ? 64 : p.split_chunk);
-        // default: split tasks above 512 vertices, or above the LDS tile when
-        // one graph's root has 4,096 vertices or more (its tasks run hundreds
-        // of branches each, and items are what spreads them over the waves;
-        // a batch of many roots has parallelism enough without)
-        const int split_min_default = count == 1 && maxR >= 4096 ? TL : 512;
+        // default: split tasks above the LDS tile when one graph is mined (its
+        // root and first levels are a handful of large tasks, and until they
+        // are split into items a few waves work while the rest wait), above
+        // 512 vertices for a batch (many roots are parallelism enough)
+        const int split_min_default = count == 1 ? TL : 512;
         A.split_min_t = (uint32_t)std::max(TL, p.split_min > 0 ? (int)p.split_min : split_min_default);
         A.cost_min_t = (uint32_t)TL;
         A.q_refs = ws.q.as<TaskRef>();

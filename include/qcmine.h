@@ -72,8 +72,8 @@ typedef struct qc_params {
     int32_t trace;           /* persistent mode: record the cost-model features of every expanded
                                 task and its subtree size (qc_result_trace) */
     int32_t split_min;       /* persistent mode: split only tasks above this many vertices (and
-                                above the LDS tile); 0 = 512, or the LDS tile for a single graph
-                                whose root has 4,096 vertices or more */
+                                above the LDS tile); 0 = the LDS tile for a single graph, 512 for
+                                a batch */
     int32_t mid_tile;        /* persistent mode: tasks above the LDS tile and up to this size
                                 (<= 4096) build task-local bit matrices in HBM; <= 0 = none (read
                                 the root matrices) */
