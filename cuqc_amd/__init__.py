@@ -82,6 +82,7 @@ def load_library() -> C.CDLL:
         "qc_graph_load_serialized": (C.c_int, [C.c_char_p, PP]),
         "qc_graph_load_adjlist": (C.c_int, [C.c_char_p, PP]),
         "qc_graph_write_serialized": (C.c_int, [P, C.c_char_p]),
+        "qc_graph_write_serialized_device": (C.c_int, [P, C.c_char_p]),
         "qc_graph_load_binary": (C.c_int, [C.c_char_p, PP]),
         "qc_graph_write_binary": (C.c_int, [P, C.c_char_p]),
         "qc_graph_from_csr": (C.c_int, [C.c_int32, P, P, PP]),
@@ -286,8 +287,12 @@ class Graph:
         _check(load_library().qc_graph_planted(self._h, C.byref(cnt), C.byref(off), C.byref(v)))
         return _sets_from(cnt.value, off, v)
 
-    def write_serialized(self, path) -> None:
-        _check(load_library().qc_graph_write_serialized(self._h, os.fsencode(path)))
+    def write_serialized(self, path, device: bool = False) -> None:
+        """The reference's serialized text format (adjToSer.cpp).  device:
+        N2 counted and built on the GPU, written in chunks (large graphs)."""
+        L = load_library()
+        fn = L.qc_graph_write_serialized_device if device else L.qc_graph_write_serialized
+        _check(fn(self._h, os.fsencode(path)))
 
     def write_binary(self, path) -> None:
         _check(load_library().qc_graph_write_binary(self._h, os.fsencode(path)))

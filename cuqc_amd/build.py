@@ -27,7 +27,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["expand.hip", "root.hip", "driver.hip", "nonmax.hip", "collect.hip", "persist.hip", "mlp.hip",
-               "capi.hip"]
+               "n2.hip", "capi.hip"]
 CXX_SOURCES = ["graph_io.cpp", "gen.cpp", "convert.cpp"]
 HEADERS = ["qc_host.h", "qc_device.h", "qc_internal.h"]
 

@@ -277,6 +277,13 @@ int qc_graph_write_serialized(const qc_graph* g, const char* path) {
     return guarded([&]() -> int { return write_serialized(g, path); });
 }
 
+int qc_graph_write_serialized_device(const qc_graph* g, const char* path) {
+    if (!g || !path) return set_error("NULL argument"), QC_EINVAL;
+    int rc = device_ok();
+    if (rc) return rc;
+    return guarded([&]() -> int { return write_serialized_device(g, path); });
+}
+
 int qc_graph_load_binary(const char* path, qc_graph** out) {
     if (!path || !out) return set_error("NULL argument"), QC_EINVAL;
     return guarded([&]() -> int {

@@ -145,6 +145,11 @@ struct Instance {
 
 // uploads the graph's arrays to HBM once (root.hip)
 int upload_graph(const qc_graph* g, hipStream_t s);
+// the graph's CSR arrays in HBM (uploaded on first use; owned by the graph)
+int graph_device_arrays(const qc_graph* g, const uint64_t** off, const uint32_t** adj, hipStream_t s);
+
+// the serialized text file with N2 built on the GPU (n2.hip)
+int write_serialized_device(const qc_graph* g, const char* path);
 
 // builds the root task and the device tables (root.hip)
 int build_instance(const qc_graph* g, double gamma, int min_size, Instance& inst, hipStream_t s);

@@ -320,6 +320,15 @@ int upload_graph(const qc_graph* g, hipStream_t s) {
     return device_csr(g, &d, s);
 }
 
+int graph_device_arrays(const qc_graph* g, const uint64_t** off, const uint32_t** adj, hipStream_t s) {
+    DevCSR* d = nullptr;
+    const int rc = device_csr(g, &d, s);
+    if (rc) return rc;
+    *off = d->off.as<uint64_t>();
+    *adj = d->adj.as<uint32_t>();
+    return QC_OK;
+}
+
 namespace {
 int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hipStream_t s) {
     const auto t_start = std::chrono::steady_clock::now();

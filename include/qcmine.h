@@ -120,6 +120,14 @@ int qc_graph_load_adjlist(const char* path, qc_graph** out);
  * full-graph N2 lists (sorted, self excluded). */
 int qc_graph_write_serialized(const qc_graph* g, const char* path);
 
+/* Replaces adjToSer.cpp:115-200 for large graphs: the same file as
+ * qc_graph_write_serialized, byte for byte, with the N2 lists counted and
+ * built on the GPU (a visited bitmap per vertex in LDS) and written in chunks,
+ * so host memory holds one chunk of N2 entries.  n is limited by the LDS a
+ * workgroup may hold (about 1.31 million vertices on gfx950; QC_EINVAL
+ * above); QC_EDEVICE without a GPU. */
+int qc_graph_write_serialized_device(const qc_graph* g, const char* path);
+
 /* Binary CSR file (header "QCG1", n, nnz, offsets u64, adjacency u32); the
  * format for graphs whose text / N2 form is impractical. */
 int qc_graph_load_binary(const char* path, qc_graph** out);
