@@ -93,8 +93,12 @@ __global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off
 
 // The same rows and counts with the visited bitmap and both rows in LDS
 // (dynamic: ceil(n/64) + 2 rw words), for graphs whose bitmap fits.  Rows are
-// written whole, so A1 / A2 need no clearing.
+// written whole, so A1 / A2 need no clearing.  NT threads share one vertex:
+// 256, or 1024 when the map leaves room for one workgroup per CU (then the
+// workgroup's waves are all the CU holds, and its loads of neighbour lists
+// are what hides the memory latency).
 constexpr int ROWS_THREADS = 256;
+constexpr int ROWS_THREADS_WIDE = 1024;
 
 __device__ __forceinline__ uint64_t rdl64(uint64_t x, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
@@ -102,8 +106,8 @@ __device__ __forceinline__ uint64_t rdl64(uint64_t x, int l) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <bool BYTES>
-__global__ __launch_bounds__(ROWS_THREADS) void root_rows_lds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+template <bool BYTES, int NT>
+__global__ __launch_bounds__(NT) void root_rows_lds(const uint64_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                                     const int32_t* __restrict__ loc, const uint32_t* __restrict__ orig,
                                                     int32_t R, int32_t rw, int32_t n, uint64_t* A1, uint64_t* A2,
                                                     int32_t* n2out) {
@@ -111,7 +115,7 @@ __global__ __launch_bounds__(ROWS_THREADS) void root_rows_lds(const uint64_t* __
     // plain stores of 1, no read-modify-write) or a bitmap (atomics without
     // return); r1: root-space row of N(v).  |N2(v)| and the A2 row are read
     // off the map afterwards.
-    // ROWS_THREADS / 64 waves share one vertex: each takes every nwv-th group
+    // NT / 64 waves share one vertex: each takes every nwv-th group
     // of neighbour lists and every nwv-th block of row words
     extern __shared__ unsigned long long sh[];
     __shared__ int s_cnt;
@@ -420,12 +424,16 @@ int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hi
     const bool byte_map = n <= 32768;
     const size_t map_words = byte_map ? ((size_t)n + 7) / 8 : ((size_t)n + 63) / 64;
     const size_t lds_bytes = (map_words + (size_t)I.rw) * 8;
-    const void* rows_kernel =
-        byte_map ? (const void*)qcdev::root_rows_lds<true> : (const void*)qcdev::root_rows_lds<false>;
     int lds_max = 0, cur_dev = 0;
     QC_HIP(hipGetDevice(&cur_dev));
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, cur_dev) != hipSuccess)
         lds_max = 64 * 1024;
+    // one workgroup per CU when two maps do not fit: then 16 waves per vertex
+    const bool rows_wide = !byte_map && lds_bytes * 2 > (size_t)lds_max;
+    const int rows_threads = rows_wide ? qcdev::ROWS_THREADS_WIDE : qcdev::ROWS_THREADS;
+    const void* rows_kernel = byte_map    ? (const void*)qcdev::root_rows_lds<true, qcdev::ROWS_THREADS>
+                              : rows_wide ? (const void*)qcdev::root_rows_lds<false, qcdev::ROWS_THREADS_WIDE>
+                                          : (const void*)qcdev::root_rows_lds<false, qcdev::ROWS_THREADS>;
     bool lds_rows = !g->has_n2 && lds_bytes <= (size_t)lds_max;
     if (lds_rows && lds_bytes > 48 * 1024 &&
         hipFuncSetAttribute(rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
@@ -466,7 +474,7 @@ int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hi
         int32_t* n2p = d_n2.as<int32_t>();
         void* rargs[] = {(void*)&offp, (void*)&adjp, (void*)&locp, (void*)&origp, (void*)&Rv,
                          (void*)&rwv,  (void*)&nv,   (void*)&A1p,  (void*)&A2p,   (void*)&n2p};
-        QC_HIP(hipLaunchKernel(rows_kernel, dim3(lblocks), dim3(qcdev::ROWS_THREADS), rargs, lds_bytes, s));
+        QC_HIP(hipLaunchKernel(rows_kernel, dim3(lblocks), dim3(rows_threads), rargs, lds_bytes, s));
     } else {
         if (!g->has_n2) {
             if ((rc = d_vis.alloc((size_t)blocks * vis_words * 8, s))) return rc;
