@@ -129,12 +129,17 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
         p.workgroups > 0 ? (uint32_t)p.workgroups : (uint32_t)std::max(1, n_cu) * 16;
 
     // ---- root ----
+    // (a window's leading removals applied at once when they are many)
+    RootCut cut;
+    if ((rc = root_window_cut(I, p.root_begin, p.root_end, cut, s))) return rc;
+    const std::vector<Vrec>& rrec = cut.cut ? cut.rec : I.root;
+    const int rt = (int)rrec.size();
     {
-        const int k = I.R <= (int)TL ? 0 : 1;
-        QC_HIP(hipMemcpyAsync(st[k].verts.p, I.root.data(), I.root.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
-        TaskRef r{0, (uint32_t)I.R, 0, 0, root_bound, 0, 0};
+        const int k = rt <= (int)TL ? 0 : 1;
+        QC_HIP(hipMemcpyAsync(st[k].verts.p, rrec.data(), rrec.size() * sizeof(Vrec), hipMemcpyHostToDevice, s));
+        TaskRef r{0, (uint32_t)rt, 0, 0, root_bound, 0, 0};
         st[k].refs.push_back(r);
-        st[k].top = I.R;
+        st[k].top = (uint64_t)rt;
     }
 
     ExpandArgs A;
@@ -161,8 +166,9 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     A.slab_mwords = (uint32_t)I.rw;
     A.stats = d_stats.as<DevStats>();
     A.error = d_err.as<int>();
-    A.g.root_b0 = p.root_begin;
-    A.g.root_b1 = p.root_end > 0 ? p.root_end : -1;
+    A.g.root_b0 = cut.b0;
+    A.g.root_b1 = cut.b1;
+    A.g.root_cut = cut.cut;
     A.g.max_depth = p.max_depth > 0 ? p.max_depth : -1;
 
     hipEvent_t e0, e1;
@@ -308,6 +314,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
             QC_HIP(hipStreamSynchronize(s));
             uint64_t bslots = 0;
             for (const TaskRef& r : batch) bslots += r.t;
+            if (out.levels == 1) bslots += (uint64_t)cut.cut;  // the root as built
             const uint64_t wrefs = st[0].refs.size() + st[1].refs.size(), wslots = st[0].top + st[1].top;
             printf("L: %llu T1: %llu %llu T2: %llu %llu B: %llu %llu C: %llu %llu\n\n", (unsigned long long)out.levels,
                    (unsigned long long)take, (unsigned long long)bslots,
@@ -335,6 +342,7 @@ int enumerate_plain(const Instance& I, const qc_params& p, EnumOut& out, hipStre
     }
     QC_HIP(hipMemcpyAsync(&out.ds, d_stats.p, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     QC_HIP(hipStreamSynchronize(s));
+    root_cut_counters(cut, I.R, TL, out.ds);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     out.kernel_ms = kms;

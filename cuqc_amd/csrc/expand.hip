@@ -1546,8 +1546,12 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
         if (ballot(f) == 0) {
             // a record holding every root vertex and no member is the whole
             // root: |N2(v) n candidates| is then |N2(v) n root| = d2r[v],
-            // with no row scans (O(R^2 / 64) words at the root otherwise)
-            const bool whole_root = nm == 0 && t0 == c.g.p->R;
+            // with no row scans (O(R^2 / 64) words at the root otherwise).
+            // A root shortened by a window's leading removals (root_cut) takes
+            // the whole root's values too: its lookahead ran on the whole
+            // root, where it failed (the host checked), so nothing is emitted
+            const bool cut_root = nm == 0 && ref.depth == 0 && c.g.p->root_cut > 0;
+            const bool whole_root = nm == 0 && (t0 == c.g.p->R || cut_root);
             if (!whole_root) clear_mask(c, w.mk0);
             long long sw = 0;
             for (int i = nm + c.lane; i < t0; i += WAVE) {
@@ -1563,7 +1567,7 @@ __device__ __attribute__((always_inline)) void process_task(Ctx& c, const TaskRe
                 if (l2 < nc - 1 || w.pin[i] + w.pex[i] < req) bad = true;
             }
             wsync();
-            if (ballot(bad) == 0) {
+            if (ballot(bad) == 0 && !cut_root) {
                 emit_clique(c, w.pid, t0);
                 ph_mark(c, PH_LOOKAHEAD);
                 return;

@@ -161,19 +161,25 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
     std::vector<Vrec> root_verts;
     std::vector<DevGraph> graphs(count);
     std::vector<float> root_feat;
+    // a window's leading removals applied at once when they are many
+    std::vector<RootCut> cuts(count);
     int maxR = 1, maxrw = 1;
     for (int i = 0; i < count; i++) {
         const Instance& I = *insts[i];
+        int crc = root_window_cut(I, params[i].root_begin, params[i].root_end, cuts[i], s);
+        if (crc) return crc;
         graphs[i] = I.dg;
-        graphs[i].root_b0 = params[i].root_begin;
-        graphs[i].root_b1 = params[i].root_end > 0 ? params[i].root_end : -1;
+        graphs[i].root_b0 = cuts[i].b0;
+        graphs[i].root_b1 = cuts[i].b1;
+        graphs[i].root_cut = cuts[i].cut;
         graphs[i].max_depth = params[i].max_depth > 0 ? params[i].max_depth : -1;
         if (I.R == 0) continue;
         maxR = std::max(maxR, I.R);
         maxrw = std::max(maxrw, I.rw);
-        TaskRef r{root_verts.size(), (uint32_t)I.R, 0, (uint16_t)i, I.root_bound, 0, 0};
+        const std::vector<Vrec>& rrec = cuts[i].cut ? cuts[i].rec : I.root;
+        TaskRef r{root_verts.size(), (uint32_t)rrec.size(), 0, (uint16_t)i, I.root_bound, 0, 0};
         roots.push_back(r);
-        root_verts.insert(root_verts.end(), I.root.begin(), I.root.end());
+        root_verts.insert(root_verts.end(), rrec.begin(), rrec.end());
         // root features (no members: indeg 0, every vertex a candidate)
         int me = INT32_MAX, mx = INT32_MIN;
         long long se = 0, s2 = 0;
@@ -602,6 +608,7 @@ int enumerate_persistent(const Instance* const* insts, const qc_params* params, 
         }
     }
     for (int i = 0; i < count; i++) {
+        root_cut_counters(cuts[i], insts[i]->R, (uint32_t)TL, first_stats[i]);
         outs[i].ds = first_stats[i];
         outs[i].kernel_ms = first_ms;
         outs[i].levels = 1;

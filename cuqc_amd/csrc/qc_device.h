@@ -120,7 +120,10 @@ struct DevGraph {
     // search window (see qc_params)
     long long root_b0, root_b1;
     int32_t max_depth;
-    int32_t pad1;
+    // root vertices a window's leading branches removed before the launch
+    // (root_window_cut): the root record then holds the rest, and its
+    // lookahead, known to fail on the whole root, is not repeated; 0: none
+    int32_t root_cut;
     // mindeg[i] = ceil(gamma (i - 1)) is evaluated from this double in the
     // kernels with the same expression as the host table (one correctly
     // rounded product, then ceil), so both give the same integers

@@ -154,6 +154,29 @@ int write_serialized_device(const qc_graph* g, const char* path);
 // builds the root task and the device tables (root.hip)
 int build_instance(const qc_graph* g, double gamma, int min_size, Instance& inst, hipStream_t s);
 
+// A window that starts at root branch b > 0 removes the root's last b
+// vertices before its first branch.  The root has no members, so none of
+// these removals can fail: root_window_cut applies them at once on the GPU
+// (many waves; inside the expansion one wave would) and returns the shorter
+// root record with the window shifted to start at 0, plus what the skipped
+// steps would have added to the counters.  cut = 0: use the root as built.
+struct RootCut {
+    int32_t cut = 0;          // root vertices removed
+    std::vector<Vrec> rec;    // the root record without them (cut > 0)
+    long long b0 = 0, b1 = -1;  // the window for that record (DevGraph root_b0 / root_b1)
+    uint64_t w_adj = 0;       // adjacency scans of the removals and of the lookahead on the removed vertices
+};
+int root_window_cut(const Instance& I, long long root_begin, long long root_end, RootCut& c, hipStream_t s);
+// the counters of a launch on a cut root, completed to those of the whole
+// root: its scans, its size, and whether it counts as a large task (t > small_max)
+inline void root_cut_counters(const RootCut& c, int32_t R, uint32_t small_max, DevStats& d) {
+    if (c.cut <= 0) return;
+    d.w_adj += c.w_adj;
+    d.sum_t += (unsigned long long)c.cut;
+    const int32_t ts = R - c.cut;
+    d.large_tasks += (unsigned long long)((uint32_t)R > small_max) - (unsigned long long)((uint32_t)ts > small_max);
+}
+
 // plain / persistent enumeration (driver.hip, persist.hip)
 struct EnumOut {
     std::vector<uint64_t> raw_off{0};

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <mutex>
 #include <thread>
 #include <cstdio>
@@ -209,6 +210,21 @@ __global__ __launch_bounds__(NT) void root_rows_lds(const uint64_t* __restrict__
         }
         __syncthreads();
         if (tid == 0) n2out[a] = s_cnt - 1;  // v itself is in the map
+    }
+}
+
+// kept root vertices against the removed ones: one wave per kept vertex,
+// cnt[i] = |N(ids[i]) n removed| from its A1 row and the removal mask
+__global__ __launch_bounds__(64) void root_cut_deg(const uint64_t* __restrict__ A1, int32_t rw,
+                                                   const uint32_t* __restrict__ ids, int32_t nk,
+                                                   const uint64_t* __restrict__ mask, int32_t* __restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    for (int i = blockIdx.x; i < nk; i += gridDim.x) {
+        const uint64_t* row = A1 + (size_t)ids[i] * rw;
+        int c = 0;
+        for (int w = lane; w < rw; w += 64) c += __popcll(row[w] & mask[w]);
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) cnt[i] = c;
     }
 }
 
@@ -576,6 +592,73 @@ int build_instance(const qc_graph* g, double gamma, int ms, Instance& I, hipStre
     // they finish before the caller may release the instance
     if (rc != QC_OK) (void)hipStreamSynchronize(s);
     return rc;
+}
+
+int root_window_cut(const Instance& I, long long root_begin, long long root_end, RootCut& c, hipStream_t s) {
+    c = RootCut();
+    c.b0 = root_begin;
+    c.b1 = root_end > 0 ? root_end : -1;
+    const int R = I.R;
+    if (R == 0) return QC_OK;
+    // the window as the expansion resolves it (branches exclude the covered
+    // block; a negative start counts from the end)
+    const long long nb = (long long)I.root_branches;
+    const long long rb0 = root_begin >= 0 ? root_begin : std::max(0ll, nb + root_begin);
+    const long long bbeg = std::min(rb0, nb);
+    long long bend = nb;
+    if (root_end > 0 && root_end < bend) bend = root_end;
+    if (bbeg <= 0 || bbeg >= bend) return QC_OK;
+    const int ts = R - (int)bbeg;
+    // worth a launch only when one wave would scan many row words (kept rows
+    // x row words; QCMINE_ROOT_CUT_WORDS overrides the 2^20 threshold)
+    long long min_words = 1ll << 20;
+    if (const char* e = getenv("QCMINE_ROOT_CUT_WORDS")) min_words = atoll(e);
+    if ((long long)ts * I.rw < min_words) return QC_OK;
+    // the lookahead of the whole root (main.cu:1349-1405) must fail, since the
+    // expansion does not repeat it on the shorter record: every vertex needs
+    // |N2 n root| >= R - 1 and indeg + exdeg >= mindeg[R] to pass
+    const int req = (int)std::ceil(I.gamma * (double)(R - 1));
+    bool passes = true;
+    for (const Vrec& v : I.root)
+        if (I.d2r[v.id] < R - 1 || v.indeg + v.exdeg < req) {
+            passes = false;
+            break;
+        }
+    if (passes) return QC_OK;
+    // removed: positions [ts, R) (branches 0 .. bbeg - 1 take them from the end)
+    std::vector<uint64_t> mask((size_t)I.rw, 0);
+    for (int q = ts; q < R; q++) {
+        const uint32_t a = (uint32_t)I.root[q].id;
+        mask[a >> 6] |= 1ull << (a & 63);
+    }
+    std::vector<uint32_t> ids((size_t)ts);
+    for (int q = 0; q < ts; q++) ids[q] = (uint32_t)I.root[q].id;
+    DBuf d_mask, d_ids, d_cnt;
+    int rc;
+    if ((rc = d_mask.alloc((size_t)I.rw * 8, s)) || (rc = d_ids.alloc((size_t)ts * 4, s)) ||
+        (rc = d_cnt.alloc((size_t)ts * 4, s)))
+        return rc;
+    QC_HIP(hipMemcpyAsync(d_mask.p, mask.data(), (size_t)I.rw * 8, hipMemcpyHostToDevice, s));
+    QC_HIP(hipMemcpyAsync(d_ids.p, ids.data(), (size_t)ts * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(qcdev::root_cut_deg, dim3(std::min(ts, 8192)), dim3(64), 0, s, I.A1.as<uint64_t>(), I.rw,
+                       d_ids.as<uint32_t>(), ts, d_mask.as<uint64_t>(), d_cnt.as<int32_t>());
+    QC_HIP(hipGetLastError());
+    std::vector<int32_t> cnt((size_t)ts), d1r((size_t)R);
+    QC_HIP(hipMemcpyAsync(cnt.data(), d_cnt.p, (size_t)ts * 4, hipMemcpyDeviceToHost, s));
+    QC_HIP(hipMemcpyAsync(d1r.data(), I.d_d1r.p, (size_t)R * 4, hipMemcpyDeviceToHost, s));
+    QC_HIP(hipStreamSynchronize(s));
+    c.rec.assign(I.root.begin(), I.root.begin() + ts);
+    for (int q = 0; q < ts; q++) c.rec[q].exdeg -= cnt[q];
+    // removal k scans N(r) of root vertex r at position R - k (main.cu:
+    // 1437-1450); the lookahead scanned N2 of every root vertex (1373-1381)
+    for (int q = ts; q < R; q++) {
+        const int a = I.root[q].id;
+        c.w_adj += (uint64_t)d1r[a] + (uint64_t)I.d2r[a];
+    }
+    c.cut = (int32_t)bbeg;
+    c.b0 = 0;
+    c.b1 = bend - bbeg;
+    return QC_OK;
 }
 
 }  // namespace qc

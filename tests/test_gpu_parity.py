@@ -88,6 +88,37 @@ def test_config0_window(cfg0, window):
         assert res.stats["beyond"] == ref["stats"]["beyond"]
 
 
+@pytest.mark.parametrize("window", [(40, 100, -1), (10, 60, 3), (150, -1, -1), (-20, -1, 2), (5, 6, -1)])
+def test_config0_window_root_cut(cfg0, window, monkeypatch):
+    """A window's leading removals applied to the root before the launch
+    (forced here for a small root) give the oracle's counters and sets."""
+    g, _ = cfg0
+    off, adj = g.csr()
+    monkeypatch.setenv("QCMINE_ROOT_CUT_WORDS", "0")
+    _say(f"[gpu] config0 window {window}, root cut forced")
+    ref = oracle.mine(off, adj, g.gamma, g.min_size, root_begin=window[0], root_end=window[1], max_depth=window[2])
+    for mode in ("plain", "persistent"):
+        res = qc.mine(g, g.gamma, g.min_size, 0, mode=mode, window=window)
+        _compare(res, ref, f"config0 window {window} {mode} cut")
+        assert res.stats["beyond"] == ref["stats"]["beyond"]
+
+
+def test_root_cut_skipped_when_the_root_is_a_quasi_clique(monkeypatch):
+    """K_30: the whole root passes the lookahead and is the one maximal set;
+    a window starting past branch 0 must not shorten it."""
+    n = 30
+    off = np.arange(0, n * (n - 1) + 1, n - 1, dtype=np.uint64)
+    adj = np.array([u for v in range(n) for u in range(n) if u != v], np.uint32)
+    g = qc.Graph.from_csr(off, adj)
+    monkeypatch.setenv("QCMINE_ROOT_CUT_WORDS", "0")
+    ref = oracle.mine(off, adj, 0.9, 10, root_begin=5, root_end=20)
+    _say("[gpu] K_30 window (5, 20), root cut forced")
+    for mode in ("plain", "persistent"):
+        res = qc.mine(g, 0.9, 10, 0, mode=mode, window=(5, 20, -1))
+        _compare(res, ref, f"K30 window {mode}")
+    assert oracle.nonmax(ref["raw"]) == [tuple(range(n))]
+
+
 @pytest.mark.parametrize("chunk", [1, 3, 16])
 def test_config0_persistent_split(cfg0, chunk):
     g, ref = cfg0

@@ -52,6 +52,20 @@ def test_workload_window_persistent(key, order):
     _check(res, gold)
 
 
+@pytest.mark.parametrize("key", [k for k in ("2", "4") if k in WINDOWS])
+def test_workload_window_plain(key):
+    """The plain form on windows that start deep in the root's branches (the
+    leading removals are applied to the root before the first level)."""
+    gold = WINDOWS[key]
+    w = workloads.WORKLOADS[int(key)]
+    g = workloads.graph(w, gold["instance"])
+    assert hex(g.checksum()) == gold["checksum"]
+    print(f"[gpu] workload {w.name} window {w.window} plain", flush=True)
+    sys.stdout.flush()
+    res = qc.mine(g, g.gamma, g.min_size, 0, mode="plain", window=w.window)
+    _check(res, gold)
+
+
 def _batch(wl):
     w = workloads.WORKLOADS[wl]
     golds = [WINDOWS[f"{wl}:{i}"] for i in range(w.instances)]
