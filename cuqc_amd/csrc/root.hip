@@ -255,25 +255,26 @@ namespace qc {
 
 namespace {
 
-// |N2(v)| over the full graph with an early exit at `cap`
+// min(|N2(v)|, cap) over the full graph: the walk stops at the cap-th
+// distinct vertex (a neighbour that is a hub would otherwise cost its whole
+// list for a vertex that passed long before)
 long n2_capped(const qc_graph* g, int v, std::vector<int>& mark, long cap) {
-    if (g->has_n2) return (long)(g->off2[v + 1] - g->off2[v]);
+    if (g->has_n2) return std::min<long>((long)(g->off2[v + 1] - g->off2[v]), cap);
     long cnt = 0;
     mark[v] = v;
     for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) {
         uint32_t u = g->adj[j];
         if (mark[u] != v) {
             mark[u] = v;
-            cnt++;
+            if (++cnt >= cap) return cnt;
         }
         for (uint64_t k = g->off[u]; k < g->off[u + 1]; k++) {
             uint32_t x = g->adj[k];
             if (mark[x] != v) {
                 mark[x] = v;
-                cnt++;
+                if (++cnt >= cap) return cnt;
             }
         }
-        if (cnt >= cap) return cnt;
     }
     return cnt;
 }
@@ -372,6 +373,7 @@ int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hi
         lab[i] = pass ? 0 : -1;
         if (pass) rem.push_back(i);
     }
+    const auto t_filter = std::chrono::steady_clock::now();
     // ---- peel (main.cu:889-949) ----
     // The reference's rounds and queue compute the core in which every
     // vertex has at least thr neighbours inside it (unique, whatever the
@@ -399,6 +401,7 @@ int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hi
             for (auto& x : th) x.join();
         }
     }
+    const auto t_count = std::chrono::steady_clock::now();
     std::vector<int32_t> queue;
     for (int v : rem)
         if (exd[v] < thr) {
@@ -521,10 +524,10 @@ int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hi
         const auto t_dev = std::chrono::steady_clock::now();
         auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
         fprintf(stderr,
-                "[qcmine] root: R=%d host filter/peel %.2f ms (from call), device rows %.2f ms (%s; uploads %.2f, "
-                "rows %.2f, degrees and tables %.2f)\n",
-                R, ms(t_start, t_host), ms(t_host, t_dev), lds_rows ? "lds" : "global", ms(t_host, t_up),
-                ms(t_up, t_rows), ms(t_rows, t_dev));
+                "[qcmine] root: R=%d host filter/peel %.2f ms (from call; filter %.2f, degrees %.2f, peel %.2f), "
+                "device rows %.2f ms (%s; uploads %.2f, rows %.2f, degrees and tables %.2f)\n",
+                R, ms(t_start, t_host), ms(t_start, t_filter), ms(t_filter, t_count), ms(t_count, t_host),
+                ms(t_host, t_dev), lds_rows ? "lds" : "global", ms(t_host, t_up), ms(t_up, t_rows), ms(t_rows, t_dev));
     }
 
     // ---- cover vertex and covered set (main.cu:953-975) ----
