@@ -100,6 +100,7 @@ __global__ __launch_bounds__(64) void root_rows(const uint64_t* __restrict__ off
 // are what hides the memory latency).
 constexpr int ROWS_THREADS = 256;
 constexpr int ROWS_THREADS_WIDE = 1024;
+constexpr int ROWS_BIG_MAX = 512;  // long neighbour lists listed per vertex (LDS)
 
 __device__ __forceinline__ uint64_t rdl64(uint64_t x, int l) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
@@ -117,9 +118,15 @@ __global__ __launch_bounds__(NT) void root_rows_lds(const uint64_t* __restrict__
     // return); r1: root-space row of N(v).  |N2(v)| and the A2 row are read
     // off the map afterwards.
     // NT / 64 waves share one vertex: each takes every nwv-th group
-    // of neighbour lists and every nwv-th block of row words
+    // of neighbour lists and every nwv-th block of row words.  Lists of NT
+    // entries or more (the hubs of a power-law graph) are walked by all the
+    // workgroup's threads together instead: in a group they kept one wave
+    // busy while the others had finished (on config 4 the busiest wave of a
+    // vertex held 5.5x the average share).
     extern __shared__ unsigned long long sh[];
     __shared__ int s_cnt;
+    __shared__ int s_nbig;
+    __shared__ uint32_t s_big[ROWS_BIG_MAX];
     const int lane = threadIdx.x & 63;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -139,7 +146,10 @@ __global__ __launch_bounds__(NT) void root_rows_lds(const uint64_t* __restrict__
         const uint32_t v = orig[a];
         for (int w = tid; w < nw; w += nt) vis[w] = 0;
         for (int w = tid; w < rw; w += nt) r1[w] = 0;
-        if (tid == 0) s_cnt = 0;
+        if (tid == 0) {
+            s_cnt = 0;
+            s_nbig = 0;
+        }
         __syncthreads();
         if (tid == 0) mark(v);
         const uint64_t b0 = off[v], b1 = off[v + 1];
@@ -148,6 +158,21 @@ __global__ __launch_bounds__(NT) void root_rows_lds(const uint64_t* __restrict__
             mark(u);
             const int la = loc[u];
             if (la >= 0) atomicOr(&r1[la >> 6], 1ull << (la & 63));
+            if (off[u + 1] - off[u] >= (uint64_t)NT) {
+                const int q = atomicAdd(&s_nbig, 1);
+                if (q < ROWS_BIG_MAX) s_big[q] = u;
+            }
+        }
+        __syncthreads();
+        // the long lists, one after another, every thread on each; when more
+        // than ROWS_BIG_MAX were found the groups below walk all long lists
+        // again (marking twice changes nothing)
+        const int nbig = min(s_nbig, ROWS_BIG_MAX);
+        const bool big_done = s_nbig <= ROWS_BIG_MAX;
+        for (int q = 0; q < nbig; q++) {
+            const uint32_t u = s_big[q];
+            const uint64_t e1 = off[u + 1];
+            for (uint64_t k = off[u] + tid; k < e1; k += nt) mark(adj[k]);
         }
         // N(u) for G neighbours u at a time as one flat range of entries:
         // lane i < G holds u_i's list start and the prefix of list lengths
@@ -157,6 +182,7 @@ __global__ __launch_bounds__(NT) void root_rows_lds(const uint64_t* __restrict__
                 const uint32_t u = adj[j0 + lane];
                 st = off[u];
                 len = off[u + 1] - st;
+                if (big_done && len >= (uint64_t)NT) len = 0;
             }
             uint64_t pre = len;  // inclusive prefix over lanes 0..G-1
             for (int o = 1; o < G; o <<= 1) {
@@ -453,7 +479,8 @@ int build_instance_impl(const qc_graph* g, double gamma, int ms, Instance& I, hi
     const void* rows_kernel = byte_map    ? (const void*)qcdev::root_rows_lds<true, qcdev::ROWS_THREADS>
                               : rows_wide ? (const void*)qcdev::root_rows_lds<false, qcdev::ROWS_THREADS_WIDE>
                                           : (const void*)qcdev::root_rows_lds<false, qcdev::ROWS_THREADS>;
-    bool lds_rows = !g->has_n2 && lds_bytes <= (size_t)lds_max;
+    // (the kernel's static LDS, the long-list table, comes on top)
+    bool lds_rows = !g->has_n2 && lds_bytes + sizeof(uint32_t) * (qcdev::ROWS_BIG_MAX + 4) <= (size_t)lds_max;
     if (lds_rows && lds_bytes > 48 * 1024 &&
         hipFuncSetAttribute(rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
         lds_rows = false;
