@@ -90,6 +90,19 @@ def test_batch_workload_one_launch(wl, order):
 
 
 @pytest.mark.parametrize("wl", BATCHES)
+def test_batch_workload_root_cut(wl, monkeypatch):
+    """The batch in one launch with every instance's leading root removals
+    applied before it (forced: these roots are below the default threshold);
+    each instance's counters must still equal its own oracle result."""
+    w, gs, golds = _batch(wl)
+    monkeypatch.setenv("QCMINE_ROOT_CUT_WORDS", "0")
+    print(f"[gpu] batch {w.name} ({len(gs)} instances), root cut forced", flush=True)
+    res = qc.mine_batch(gs, [g.gamma for g in gs], [g.min_size for g in gs], windows=[w.window] * len(gs))
+    for r, gold in zip(res, golds):
+        _check(r, gold)
+
+
+@pytest.mark.parametrize("wl", BATCHES)
 def test_batch_workload_plain_subset(wl):
     """Every 16th instance on its own in the plain level-synchronous form."""
     w, gs, golds = _batch(wl)
