@@ -9,8 +9,9 @@ it):
 * ``lds_tile``    tasks up to this many vertices run from LDS (64/128/256);
 * ``split_chunk`` branch-range work items of this many branches (-1: never
                   split);
-* ``split_min``   split only tasks above this many vertices (values below the
-                  LDS tile mean the tile);
+* ``split_min``   split only tasks above this many vertices (0: the library
+                  default, the LDS tile for a single graph and 512 for a
+                  batch; values below the LDS tile mean the tile);
 * ``mid_tile``    tasks above the LDS tile and up to this size build task-local
                   bit matrices in HBM (0: none, read the root matrices);
 * ``lds_wide``    tasks above the LDS tile and up to this size keep their
@@ -37,13 +38,13 @@ from dataclasses import asdict, dataclass, field
 
 SPACE: dict[str, list[int]] = {
     "split_chunk": [16, 32, 64, 128, -1],
-    "split_min": [1, 512, 1024],
+    "split_min": [0, 1, 512, 1024],
     "mid_tile": [0, 512, 1024],
     "lds_wide": [-1, 256, 512, 1024],
     "lds_tile": [64, 128, 256],
     "workgroups": [1024, 2048, 3072, 0],
 }
-DEFAULT = {"lds_tile": 128, "split_chunk": 64, "split_min": 512, "mid_tile": 0, "lds_wide": 0, "workgroups": 0}
+DEFAULT = {"lds_tile": 128, "split_chunk": 64, "split_min": 0, "mid_tile": 0, "lds_wide": 0, "workgroups": 0}
 ORDER = ("split_chunk", "lds_wide", "split_min", "mid_tile", "lds_tile", "workgroups")
 
 _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
@@ -53,7 +54,7 @@ _INVARIANT = ("tasks", "branches", "children", "raw_cliques", "w_adj", "beyond")
 class Schedule:
     lds_tile: int = 128
     split_chunk: int = 64
-    split_min: int = 512
+    split_min: int = 0
     mid_tile: int = 0
     lds_wide: int = 0
     workgroups: int = 0
