@@ -152,6 +152,15 @@ def test_mining_without_device_fails_loudly():
     assert e.value.code == qc.QC_EDEVICE
 
 
+@pytest.mark.skipif(qc.device_available(), reason="checks the no-device error path")
+def test_device_n2_writer_without_device_fails_loudly(tmp_path):
+    g = qc.Graph.generate(0, 1)
+    with pytest.raises(qc.QCError) as e:
+        g.write_serialized(tmp_path / "g.txt", device=True)
+    assert e.value.code == qc.QC_EDEVICE
+    assert not (tmp_path / "g.txt").exists()
+
+
 def _cli(*args, cwd=None):
     r = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=120)
     return r.returncode, r.stdout
