@@ -89,6 +89,45 @@ def test_serialized_round_trip(tmp_path):
     assert np.array_equal(o2, off) and np.array_equal(a2, adj)
 
 
+def _csr_from_pairs(n, pairs):
+    nbr = [set() for _ in range(n)]
+    for u, v in pairs:
+        if u != v:
+            nbr[u].add(v)
+            nbr[v].add(u)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(s) for s in nbr])
+    adj = np.array([x for s in nbr for x in sorted(s)], np.uint32)
+    return off, adj
+
+
+@pytest.mark.parametrize("case", ["hub_isolated", "n70", "sparse"])
+def test_serialized_n2_records_equal_the_oracle(tmp_path, case):
+    """Both N2 records of the serialized file (lists and offsets) against the
+    oracle's N2 builder, itself pinned against a dense A + A^2.  The device
+    writer is compared byte for byte with this file on the GPU."""
+    rng = np.random.default_rng(11)
+    if case == "hub_isolated":
+        n = 300
+        pairs = [(0, int(v)) for v in rng.choice(np.arange(1, 290), 150, replace=False)]
+        pairs += [tuple(int(x) for x in rng.integers(0, 290, 2)) for _ in range(400)]
+    elif case == "n70":
+        n = 70
+        pairs = [(i, (i * 7 + 3) % 70) for i in range(70)]
+    else:
+        n = 500
+        pairs = [tuple(int(x) for x in rng.integers(0, n, 2)) for _ in range(300)]
+    off, adj = _csr_from_pairs(n, pairs)
+    g = qc.Graph.from_csr(off, adj)
+    p = tmp_path / "g.txt"
+    g.write_serialized(p)
+    lines = p.read_text().split("\n")
+    off2, adj2 = oracle.build_n2(off, adj)
+    assert int(lines[2]) == len(adj2)
+    assert [int(x) for x in lines[5].split()] == [int(x) for x in adj2]
+    assert [int(x) for x in lines[6].split()] == [int(x) for x in off2]
+
+
 def test_binary_round_trip(tmp_path):
     off, adj = planted(80, 0.1, [9], 0.9, 4)
     g = qc.Graph.from_csr(off, adj)
